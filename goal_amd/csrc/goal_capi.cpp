@@ -1,0 +1,1089 @@
+// C ABI of goal_amd (declared in include/goal_amd.h).  Host C++ that owns the
+// device buffers, builds meshes and graphs, and queues the kernels of
+// goal_assembly.hip / goal_util.hip / goal_dwr.hip on one HIP stream.
+#include "goal_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "goal_expr.hpp"
+#include "goal_kernels.hpp"
+#include "goal_mesh.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+struct HipError : std::runtime_error {
+  int code;
+  HipError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    throw HipError(GOAL_ERR_ALLOC, std::string(what) + ": " + hipGetErrorString(e));
+  throw HipError(GOAL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+// Every entry point runs its body through guard(): exceptions become status
+// codes plus a message, never an abort.
+template <typename F>
+int guard(F&& f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return set_error(e.code, "%s", e.what());
+  } catch (const std::invalid_argument& e) {
+    return set_error(GOAL_ERR_INVALID, "%s", e.what());
+  } catch (const std::bad_alloc&) {
+    return set_error(GOAL_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception& e) {
+    return set_error(GOAL_ERR_INVALID, "%s", e.what());
+  }
+}
+
+void require(bool ok, const char* msg) {
+  if (!ok) throw std::invalid_argument(msg);
+}
+
+// ---------------------------------------------------------------- context
+struct Context {
+  bool ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t events[256] = {};
+  double* scratch = nullptr;  // reductions: kReduceBlocks partials + results
+  std::mutex mu;
+};
+Context g_ctx;
+
+Context& ctx() {
+  if (g_ctx.ready) return g_ctx;
+  std::lock_guard<std::mutex> lock(g_ctx.mu);
+  if (g_ctx.ready) return g_ctx;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) throw HipError(GOAL_ERR_NO_DEVICE, "no HIP device available");
+  HIPCHK(hipGetDevice(&g_ctx.device));
+  HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  for (auto& ev : g_ctx.events) HIPCHK(hipEventCreate(&ev));
+  HIPCHK(hipMalloc(&g_ctx.scratch, sizeof(double) * (goal::kReduceBlocks + 16)));
+  g_ctx.ready = true;
+  return g_ctx;
+}
+
+hipStream_t stream() { return ctx().stream; }
+double* scratch() { return ctx().scratch; }
+double* result_slot(int k) { return ctx().scratch + goal::kReduceBlocks + k; }
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  int64_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(int64_t count) {
+    release();
+    if (count <= 0) return;
+    HIPCHK(hipMalloc(&p, sizeof(T) * count));
+    n = count;
+  }
+  void upload(const T* host, int64_t count) {
+    alloc(count);
+    if (count > 0) HIPCHK(hipMemcpyAsync(p, host, sizeof(T) * count, hipMemcpyHostToDevice, stream()));
+  }
+  void download(T* host, int64_t count) const {
+    if (count > 0) {
+      HIPCHK(hipMemcpyAsync(host, p, sizeof(T) * count, hipMemcpyDeviceToHost, stream()));
+      HIPCHK(hipStreamSynchronize(stream()));
+    }
+  }
+};
+
+void sync() { HIPCHK(hipStreamSynchronize(stream())); }
+
+int host_threads() {
+  unsigned t = std::thread::hardware_concurrency();
+  if (t < 1) t = 1;
+  return (int)std::min<unsigned>(t, 32);
+}
+
+goal::MatParams derive(const goal_material& m) {
+  goal::MatParams p;
+  p.mu = m.E / (2.0 * (1.0 + m.nu));
+  p.kappa = m.E / (3.0 * (1.0 - 2.0 * m.nu));
+  p.lambda = m.E * m.nu / ((1.0 + m.nu) * (1.0 - 2.0 * m.nu));
+  p.K = m.K;
+  p.Y = m.Y;
+  p.c0 = m.c0;
+  return p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- handles
+struct goal_disc_s {
+  goal::HostMesh mesh;
+  std::vector<int64_t> node_adj_ptr;
+  int max_degree = 0;
+  int64_t num_dofs = 0, nnz = 0;
+  DevBuf<double> coords, x;
+  DevBuf<int32_t> conn, elem_set, cols, vertex_nodes;
+  DevBuf<uint16_t> pos;
+  DevBuf<int64_t> row_ptr;
+  std::map<std::string, std::unique_ptr<DevBuf<int32_t>>> dev_node_sets;
+  DevBuf<double> dbc_values;
+};
+
+struct goal_states_s {
+  goal_disc d = nullptr;
+  int model = 0, nq = 1;
+  int64_t npts = 0;
+  DevBuf<double> sigma, eqps, eqps_old, Fp, Fp_old;
+};
+
+struct goal_solinfo_s {
+  goal_disc d = nullptr;
+  DevBuf<double> R, dMdu, vals;
+};
+
+struct goal_mech_s {
+  goal_disc d = nullptr;
+  goal_states st = nullptr;
+  int model = 0, stab = 1, quad_order = 1, num_sets = 1;
+  int blocks = 0, xcd_remap = 1, check = 1;
+  DevBuf<goal::MatParams> mats;
+  DevBuf<int32_t> status;
+  DevBuf<double> elem_value;
+};
+
+struct goal_nested_s {
+  goal_disc base = nullptr;
+  goal_disc fine = nullptr;
+  int64_t num_old = 0, num_new = 0;
+  std::vector<int32_t> parent_host, edge_parents_host;
+  DevBuf<int32_t> parent, edge_parents;
+};
+
+namespace {
+
+void disc_build(goal_disc d) {
+  goal::HostGraph g = goal::build_graph(d->mesh, host_threads());
+  d->node_adj_ptr = g.node_adj_ptr;
+  d->max_degree = g.max_degree;
+  d->num_dofs = g.num_dofs;
+  d->nnz = g.nnz;
+  const goal::HostMesh& m = d->mesh;
+  d->coords.upload(m.coords.data(), (int64_t)m.coords.size());
+  d->conn.upload(m.conn.data(), (int64_t)m.conn.size());
+  d->elem_set.upload(m.elem_set.data(), (int64_t)m.elem_set.size());
+  d->vertex_nodes.upload(m.vertex_nodes.data(), (int64_t)m.vertex_nodes.size());
+  d->pos.upload(g.pos.data(), (int64_t)g.pos.size());
+  DevBuf<int64_t> adj_ptr;
+  DevBuf<int32_t> adj;
+  adj_ptr.upload(g.node_adj_ptr.data(), (int64_t)g.node_adj_ptr.size());
+  adj.upload(g.node_adj.data(), (int64_t)g.node_adj.size());
+  d->row_ptr.alloc(d->num_dofs + 1);
+  d->cols.alloc(d->nnz);
+  HIPCHK(goal::launch_expand_csr(adj_ptr.p, adj.p, m.num_nodes, d->row_ptr.p, d->cols.p, stream()));
+  d->x.alloc(d->num_dofs);
+  HIPCHK(goal::launch_fill(d->x.p, d->num_dofs, 0.0, stream()));
+  sync();  // adj buffers are released on return
+}
+
+goal_disc new_disc(goal::HostMesh&& m) {
+  std::unique_ptr<goal_disc_s> d(new goal_disc_s);
+  d->mesh = std::move(m);
+  disc_build(d.get());
+  return d.release();
+}
+
+const DevBuf<int32_t>& dev_node_set(goal_disc d, const std::string& name) {
+  auto it = d->mesh.node_sets.find(name);
+  if (it == d->mesh.node_sets.end()) throw std::invalid_argument("unknown node set: " + name);
+  auto& slot = d->dev_node_sets[name];
+  if (!slot) {
+    slot.reset(new DevBuf<int32_t>);
+    slot->upload(it->second.data(), (int64_t)it->second.size());
+  }
+  return *slot;
+}
+
+int nq_of(int quad_order) { return quad_order == 1 ? 1 : 4; }
+
+int finish_assembly(goal_mech m) {
+  if (!m->check) return GOAL_OK;
+  int flags = 0;
+  HIPCHK(hipMemcpyAsync(&flags, m->status.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  sync();
+  if (flags) {
+    HIPCHK(hipMemsetAsync(m->status.p, 0, sizeof(int), stream()));
+    if (flags & 1) return set_error(GOAL_ERR_J2_RETURN_MAP, "J2: return mapping failed");
+  }
+  return GOAL_OK;
+}
+
+goal::AsmArgs asm_args(goal_mech m) {
+  goal::AsmArgs a{};
+  goal_disc d = m->d;
+  a.num_elems = d->mesh.num_elems;
+  a.conn = d->conn.p;
+  a.coords = d->coords.p;
+  a.x = d->x.p;
+  a.elem_set = d->elem_set.p;
+  a.mats = m->mats.p;
+  a.row_ptr = d->row_ptr.p;
+  a.pos = d->pos.p;
+  a.model = m->model;
+  a.stabilization = m->stab;
+  a.xcd_remap = m->xcd_remap;
+  a.status = m->status.p;
+  goal_states st = m->st;
+  a.sigma = st->sigma.p;
+  a.eqps = st->eqps.p;
+  a.eqps_old = st->eqps_old.p;
+  a.Fp = st->Fp.p;
+  a.Fp_old = st->Fp_old.p;
+  return a;
+}
+
+int asm_blocks(goal_mech m) {
+  if (m->blocks > 0) return m->blocks;
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx().device);
+  const int epw = m->d->mesh.order == 1 ? 4 : 1;
+  int64_t groups = (m->d->mesh.num_elems + epw - 1) / epw;
+  int64_t want = (int64_t)cus * 24;
+  want = std::min<int64_t>(want, std::max<int64_t>(groups, 1));
+  want = std::max<int64_t>(8, (want + 7) / 8 * 8);
+  return (int)want;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* goal_last_error(void) { return g_err.c_str(); }
+const char* goal_version(void) { return "goal_amd 0.1 (gfx950)"; }
+
+int goal_device_alloc(int64_t bytes, void** ptr) {
+  return guard([&]() -> int {
+    require(ptr && bytes >= 0, "goal_device_alloc: bad arguments");
+    ctx();
+    *ptr = nullptr;
+    if (bytes > 0) HIPCHK(hipMalloc(ptr, bytes));
+    return GOAL_OK;
+  });
+}
+int goal_device_free(void* p) {
+  return guard([&]() -> int {
+    if (p) HIPCHK(hipFree(p));
+    return GOAL_OK;
+  });
+}
+int goal_memcpy_h2d(void* dst, const void* src, int64_t bytes) {
+  return guard([&]() -> int {
+    if (bytes > 0) {
+      HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+      sync();
+    }
+    return GOAL_OK;
+  });
+}
+int goal_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
+  return guard([&]() -> int {
+    if (bytes > 0) {
+      HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+      sync();
+    }
+    return GOAL_OK;
+  });
+}
+int goal_sync(void) {
+  return guard([&]() -> int {
+    sync();
+    return GOAL_OK;
+  });
+}
+int goal_event_record(int slot) {
+  return guard([&]() -> int {
+    require(slot >= 0 && slot < 256, "event slot out of range");
+    HIPCHK(hipEventRecord(ctx().events[slot], stream()));
+    return GOAL_OK;
+  });
+}
+int goal_event_elapsed(int a, int b, float* ms) {
+  return guard([&]() -> int {
+    require(a >= 0 && a < 256 && b >= 0 && b < 256 && ms, "event slot out of range");
+    HIPCHK(hipEventSynchronize(ctx().events[b]));
+    HIPCHK(hipEventElapsedTime(ms, ctx().events[a], ctx().events[b]));
+    return GOAL_OK;
+  });
+}
+int goal_device_copy(double* dst, const double* src, int64_t n) {
+  return guard([&]() -> int {
+    require(dst && src && n >= 0, "goal_device_copy: bad arguments");
+    HIPCHK(goal::launch_copy(dst, src, n, stream()));
+    return GOAL_OK;
+  });
+}
+int goal_device_info(char* name, int len, int* num_cus) {
+  return guard([&]() -> int {
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, ctx().device));
+    if (name && len > 0) {
+      std::snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (num_cus) *num_cus = p.multiProcessorCount;
+    return GOAL_OK;
+  });
+}
+
+// ------------------------------------------------------------- host mesh
+int goal_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, int order, double jitter, uint64_t seed,
+                  int64_t* sizes, double* coords, int32_t* conn) {
+  return guard([&]() -> int {
+    require(sizes != nullptr, "goal_mesh_box: null sizes");
+    goal::HostMesh m = goal::make_box_mesh(nx, ny, nz, lx, ly, lz, order, jitter, seed);
+    sizes[0] = m.num_nodes;
+    sizes[1] = m.num_elems;
+    if (coords) std::memcpy(coords, m.coords.data(), sizeof(double) * m.coords.size());
+    if (conn) std::memcpy(conn, m.conn.data(), sizeof(int32_t) * m.conn.size());
+    return GOAL_OK;
+  });
+}
+
+int goal_mesh_graph(int order, int64_t num_nodes, int64_t num_elems, const int32_t* conn, int64_t* nnz,
+                    int64_t* row_ptr, int32_t* cols) {
+  return guard([&]() -> int {
+    require(conn && nnz && num_nodes > 0 && num_elems > 0, "goal_mesh_graph: bad arguments");
+    goal::HostMesh m;
+    m.order = order;
+    m.nodes_per_elem = order == 1 ? 4 : 10;
+    m.num_nodes = num_nodes;
+    m.num_elems = num_elems;
+    m.coords.assign(num_nodes * 3, 0.0);
+    m.conn.assign(conn, conn + num_elems * m.nodes_per_elem);
+    goal::finalize_mesh(m);
+    goal::HostGraph g = goal::build_graph(m, host_threads());
+    *nnz = g.nnz;
+    if (row_ptr && cols) {
+      std::vector<int64_t> rp;
+      std::vector<int32_t> cl;
+      goal::expand_csr(m, g, rp, cl);
+      std::memcpy(row_ptr, rp.data(), sizeof(int64_t) * rp.size());
+      std::memcpy(cols, cl.data(), sizeof(int32_t) * cl.size());
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_mesh_refine(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
+                     int64_t* sizes, double* fine_coords, int32_t* fine_conn, int32_t* parent,
+                     int32_t* edge_parents) {
+  return guard([&]() -> int {
+    require(coords && conn && sizes && num_nodes > 0 && num_elems > 0, "goal_mesh_refine: bad arguments");
+    goal::HostMesh m;
+    m.order = 1;
+    m.nodes_per_elem = 4;
+    m.num_nodes = num_nodes;
+    m.num_elems = num_elems;
+    m.coords.assign(coords, coords + num_nodes * 3);
+    m.conn.assign(conn, conn + num_elems * 4);
+    goal::finalize_mesh(m);
+    goal::Refinement r = goal::refine_uniform(m);
+    sizes[0] = r.fine.num_nodes;
+    sizes[1] = r.fine.num_elems;
+    sizes[2] = (int64_t)r.edge_parents.size() / 2;
+    if (fine_coords) std::memcpy(fine_coords, r.fine.coords.data(), sizeof(double) * r.fine.coords.size());
+    if (fine_conn) std::memcpy(fine_conn, r.fine.conn.data(), sizeof(int32_t) * r.fine.conn.size());
+    if (parent) std::memcpy(parent, r.parent.data(), sizeof(int32_t) * r.parent.size());
+    if (edge_parents) std::memcpy(edge_parents, r.edge_parents.data(), sizeof(int32_t) * r.edge_parents.size());
+    return GOAL_OK;
+  });
+}
+
+// ------------------------------------------------------------------ disc
+int goal_disc_create(int order, int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
+                     const int32_t* elem_set, goal_disc* out) {
+  return guard([&]() -> int {
+    require(out && coords && conn && num_nodes > 0 && num_elems > 0, "goal_disc_create: bad arguments");
+    ctx();
+    goal::HostMesh m;
+    m.order = order;
+    m.nodes_per_elem = order == 1 ? 4 : 10;
+    m.num_nodes = num_nodes;
+    m.num_elems = num_elems;
+    m.coords.assign(coords, coords + num_nodes * 3);
+    m.conn.assign(conn, conn + num_elems * m.nodes_per_elem);
+    if (elem_set) m.elem_set.assign(elem_set, elem_set + num_elems);
+    goal::finalize_mesh(m);
+    *out = new_disc(std::move(m));
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_create_box(int nx, int ny, int nz, double lx, double ly, double lz, int order, double jitter,
+                         uint64_t seed, goal_disc* out) {
+  return guard([&]() -> int {
+    require(out != nullptr, "goal_disc_create_box: null output");
+    ctx();
+    *out = new_disc(goal::make_box_mesh(nx, ny, nz, lx, ly, lz, order, jitter, seed));
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_destroy(goal_disc d) {
+  return guard([&]() -> int {
+    if (d) {
+      sync();
+      delete d;
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_sizes(goal_disc d, int64_t* out) {
+  return guard([&]() -> int {
+    require(d && out, "goal_disc_sizes: bad arguments");
+    out[0] = d->mesh.num_nodes;
+    out[1] = d->mesh.num_elems;
+    out[2] = d->num_dofs;
+    out[3] = d->nnz;
+    out[4] = d->mesh.nodes_per_elem;
+    out[5] = (int64_t)d->mesh.vertex_nodes.size();
+    out[6] = d->mesh.order;
+    out[7] = d->mesh.num_elem_sets;
+    out[8] = d->max_degree;
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_get_mesh(goal_disc d, double* coords, int32_t* conn, int32_t* elem_set) {
+  return guard([&]() -> int {
+    require(d != nullptr, "null disc");
+    if (coords) std::memcpy(coords, d->mesh.coords.data(), sizeof(double) * d->mesh.coords.size());
+    if (conn) std::memcpy(conn, d->mesh.conn.data(), sizeof(int32_t) * d->mesh.conn.size());
+    if (elem_set) std::memcpy(elem_set, d->mesh.elem_set.data(), sizeof(int32_t) * d->mesh.elem_set.size());
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_get_graph(goal_disc d, int64_t* row_ptr, int32_t* cols) {
+  return guard([&]() -> int {
+    require(d != nullptr, "null disc");
+    if (row_ptr) d->row_ptr.download(row_ptr, d->num_dofs + 1);
+    if (cols) d->cols.download(cols, d->nnz);
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_get_lids(goal_disc d, int32_t* lids) {
+  return guard([&]() -> int {
+    require(d && lids, "goal_disc_get_lids: bad arguments");
+    const int npe = d->mesh.nodes_per_elem;
+    for (int64_t e = 0; e < d->mesh.num_elems; ++e)
+      for (int n = 0; n < npe; ++n)
+        for (int eq = 0; eq < goal::kNumEqs; ++eq)
+          lids[(e * npe + n) * goal::kNumEqs + eq] = d->mesh.conn[e * npe + n] * goal::kNumEqs + eq;
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_add_node_set(goal_disc d, const char* name, int64_t n, const int32_t* nodes) {
+  return guard([&]() -> int {
+    require(d && name && n >= 0 && (n == 0 || nodes), "goal_disc_add_node_set: bad arguments");
+    std::vector<int32_t> v(nodes, nodes + n);
+    for (int32_t x : v) require(x >= 0 && x < d->mesh.num_nodes, "node set entry out of range");
+    d->mesh.node_sets[name] = std::move(v);
+    d->dev_node_sets.erase(name);
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_node_set_size(goal_disc d, const char* name, int64_t* n) {
+  return guard([&]() -> int {
+    require(d && name && n, "bad arguments");
+    auto it = d->mesh.node_sets.find(name);
+    if (it == d->mesh.node_sets.end()) return set_error(GOAL_ERR_INVALID, "unknown node set: %s", name);
+    *n = (int64_t)it->second.size();
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_get_node_set(goal_disc d, const char* name, int32_t* nodes) {
+  return guard([&]() -> int {
+    require(d && name && nodes, "bad arguments");
+    auto it = d->mesh.node_sets.find(name);
+    if (it == d->mesh.node_sets.end()) return set_error(GOAL_ERR_INVALID, "unknown node set: %s", name);
+    std::copy(it->second.begin(), it->second.end(), nodes);
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_set_elem_sets(goal_disc d, int num_sets, const int32_t* elem_set) {
+  return guard([&]() -> int {
+    require(d && elem_set && num_sets >= 1, "goal_disc_set_elem_sets: bad arguments");
+    for (int64_t e = 0; e < d->mesh.num_elems; ++e)
+      require(elem_set[e] >= 0 && elem_set[e] < num_sets, "element set id out of range");
+    d->mesh.elem_set.assign(elem_set, elem_set + d->mesh.num_elems);
+    d->mesh.num_elem_sets = num_sets;
+    d->elem_set.upload(d->mesh.elem_set.data(), d->mesh.num_elems);
+    sync();
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_set_soln(goal_disc d, const double* x) {
+  return guard([&]() -> int {
+    require(d && x, "bad arguments");
+    HIPCHK(hipMemcpyAsync(d->x.p, x, sizeof(double) * d->num_dofs, hipMemcpyHostToDevice, stream()));
+    sync();
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_get_soln(goal_disc d, double* x) {
+  return guard([&]() -> int {
+    require(d && x, "bad arguments");
+    d->x.download(x, d->num_dofs);
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_add_soln(goal_disc d, const double* du) {
+  return guard([&]() -> int {
+    require(d && du, "bad arguments");
+    DevBuf<double> tmp;
+    tmp.upload(du, d->num_dofs);
+    HIPCHK(goal::launch_axpy(d->x.p, tmp.p, 1.0, d->num_dofs, stream()));
+    sync();
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_add_soln_dev(goal_disc d, const double* du_dev) {
+  return guard([&]() -> int {
+    require(d && du_dev, "bad arguments");
+    HIPCHK(goal::launch_axpy(d->x.p, du_dev, 1.0, d->num_dofs, stream()));
+    return GOAL_OK;
+  });
+}
+
+double* goal_disc_soln_dev(goal_disc d) { return d ? d->x.p : nullptr; }
+
+// ---------------------------------------------------------------- states
+int goal_states_create(goal_disc d, int model, int quad_order, goal_states* out) {
+  return guard([&]() -> int {
+    require(d && out, "goal_states_create: bad arguments");
+    require(model >= 0 && model <= 2, "unknown model");
+    require(quad_order == 1 || quad_order == 2, "quadrature order must be 1 or 2");
+    std::unique_ptr<goal_states_s> s(new goal_states_s);
+    s->d = d;
+    s->model = model;
+    s->nq = nq_of(quad_order);
+    s->npts = d->mesh.num_elems * s->nq;
+    s->sigma.alloc(9 * s->npts);
+    HIPCHK(goal::launch_fill(s->sigma.p, 9 * s->npts, 0.0, stream()));
+    if (model == GOAL_MODEL_J2) {
+      s->eqps.alloc(s->npts);
+      s->eqps_old.alloc(s->npts);
+      s->Fp.alloc(9 * s->npts);
+      s->Fp_old.alloc(9 * s->npts);
+      HIPCHK(goal::launch_fill(s->eqps.p, s->npts, 0.0, stream()));
+      HIPCHK(goal::launch_fill(s->eqps_old.p, s->npts, 0.0, stream()));
+      for (int k = 0; k < 9; ++k) {
+        const double v = (k % 4 == 0) ? 1.0 : 0.0;
+        HIPCHK(goal::launch_fill(s->Fp.p + k * s->npts, s->npts, v, stream()));
+        HIPCHK(goal::launch_fill(s->Fp_old.p + k * s->npts, s->npts, v, stream()));
+      }
+    }
+    sync();
+    *out = s.release();
+    return GOAL_OK;
+  });
+}
+
+int goal_states_destroy(goal_states s) {
+  return guard([&]() -> int {
+    if (s) {
+      sync();
+      delete s;
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_states_update(goal_states s) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null states");
+    if (s->model == GOAL_MODEL_J2) {
+      HIPCHK(goal::launch_copy(s->eqps_old.p, s->eqps.p, s->npts, stream()));
+      HIPCHK(goal::launch_copy(s->Fp_old.p, s->Fp.p, 9 * s->npts, stream()));
+    }
+    return GOAL_OK;
+  });
+}
+
+static DevBuf<double>* state_field(goal_states s, const char* name, int* ncomp) {
+  std::string n(name);
+  *ncomp = 9;
+  if (n == "sigma") return &s->sigma;
+  if (s->model == GOAL_MODEL_J2) {
+    if (n == "Fp") return &s->Fp;
+    if (n == "Fp_old") return &s->Fp_old;
+    *ncomp = 1;
+    if (n == "eqps") return &s->eqps;
+    if (n == "eqps_old") return &s->eqps_old;
+  }
+  throw std::invalid_argument("unknown state field: " + n);
+}
+
+int goal_states_get(goal_states s, const char* name, double* out) {
+  return guard([&]() -> int {
+    require(s && name && out, "bad arguments");
+    int nc;
+    DevBuf<double>* f = state_field(s, name, &nc);
+    f->download(out, nc * s->npts);
+    return GOAL_OK;
+  });
+}
+
+int goal_states_set(goal_states s, const char* name, const double* in) {
+  return guard([&]() -> int {
+    require(s && name && in, "bad arguments");
+    int nc;
+    DevBuf<double>* f = state_field(s, name, &nc);
+    HIPCHK(hipMemcpyAsync(f->p, in, sizeof(double) * nc * s->npts, hipMemcpyHostToDevice, stream()));
+    sync();
+    return GOAL_OK;
+  });
+}
+
+// ---------------------------------------------------------------- solinfo
+int goal_solinfo_create(goal_disc d, goal_solinfo* out) {
+  return guard([&]() -> int {
+    require(d && out, "bad arguments");
+    std::unique_ptr<goal_solinfo_s> s(new goal_solinfo_s);
+    s->d = d;
+    s->R.alloc(d->num_dofs);
+    s->dMdu.alloc(d->num_dofs);
+    s->vals.alloc(d->nnz);
+    HIPCHK(goal::launch_fill(s->R.p, d->num_dofs, 0.0, stream()));
+    HIPCHK(goal::launch_fill(s->dMdu.p, d->num_dofs, 0.0, stream()));
+    HIPCHK(goal::launch_fill(s->vals.p, d->nnz, 0.0, stream()));
+    sync();
+    *out = s.release();
+    return GOAL_OK;
+  });
+}
+
+int goal_solinfo_destroy(goal_solinfo s) {
+  return guard([&]() -> int {
+    if (s) {
+      sync();
+      delete s;
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_solinfo_zero_R(goal_solinfo s) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solinfo");
+    HIPCHK(goal::launch_fill(s->R.p, s->R.n, 0.0, stream()));
+    return GOAL_OK;
+  });
+}
+int goal_solinfo_zero_dMdu(goal_solinfo s) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solinfo");
+    HIPCHK(goal::launch_fill(s->dMdu.p, s->dMdu.n, 0.0, stream()));
+    return GOAL_OK;
+  });
+}
+int goal_solinfo_zero_dRdu(goal_solinfo s) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solinfo");
+    HIPCHK(goal::launch_fill(s->vals.p, s->vals.n, 0.0, stream()));
+    return GOAL_OK;
+  });
+}
+int goal_solinfo_zero_all(goal_solinfo s) {
+  int st = goal_solinfo_zero_R(s);
+  if (st) return st;
+  st = goal_solinfo_zero_dRdu(s);
+  if (st) return st;
+  return goal_solinfo_zero_dMdu(s);
+}
+
+int goal_solinfo_get(goal_solinfo s, double* R, double* dMdu, double* vals) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solinfo");
+    if (R) s->R.download(R, s->R.n);
+    if (dMdu) s->dMdu.download(dMdu, s->dMdu.n);
+    if (vals) s->vals.download(vals, s->vals.n);
+    return GOAL_OK;
+  });
+}
+
+int goal_solinfo_set(goal_solinfo s, const double* R, const double* dMdu, const double* vals) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solinfo");
+    if (R) HIPCHK(hipMemcpyAsync(s->R.p, R, sizeof(double) * s->R.n, hipMemcpyHostToDevice, stream()));
+    if (dMdu) HIPCHK(hipMemcpyAsync(s->dMdu.p, dMdu, sizeof(double) * s->dMdu.n, hipMemcpyHostToDevice, stream()));
+    if (vals) HIPCHK(hipMemcpyAsync(s->vals.p, vals, sizeof(double) * s->vals.n, hipMemcpyHostToDevice, stream()));
+    sync();
+    return GOAL_OK;
+  });
+}
+
+int goal_solinfo_dev(goal_solinfo s, double** R, double** dMdu, double** vals) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solinfo");
+    if (R) *R = s->R.p;
+    if (dMdu) *dMdu = s->dMdu.p;
+    if (vals) *vals = s->vals.p;
+    return GOAL_OK;
+  });
+}
+
+int goal_solinfo_norm_R(goal_solinfo s, double* norm) {
+  return guard([&]() -> int {
+    require(s && norm, "bad arguments");
+    HIPCHK(goal::launch_reduce(1, s->R.p, nullptr, s->R.n, scratch(), result_slot(0), stream()));
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, result_slot(0), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    sync();
+    *norm = std::sqrt(v);
+    return GOAL_OK;
+  });
+}
+
+int goal_solinfo_dot_R(goal_solinfo s, const double* z_dev, double* dot) {
+  return guard([&]() -> int {
+    require(s && z_dev && dot, "bad arguments");
+    HIPCHK(goal::launch_reduce(2, s->R.p, z_dev, s->R.n, scratch(), result_slot(0), stream()));
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, result_slot(0), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    sync();
+    *dot = -v;
+    return GOAL_OK;
+  });
+}
+
+// ------------------------------------------------------------- mechanics
+int goal_mech_create(goal_disc d, goal_states st, int num_sets, const goal_material* mats, int stabilization,
+                     int quad_order, goal_mech* out) {
+  return guard([&]() -> int {
+    require(d && st && mats && out && num_sets >= 1, "goal_mech_create: bad arguments");
+    require(st->d == d, "states belong to another disc");
+    require(quad_order == 1 || quad_order == 2, "quadrature order must be 1 or 2");
+    require(nq_of(quad_order) == st->nq, "states were created for another quadrature order");
+    require(num_sets >= d->mesh.num_elem_sets, "fewer materials than element sets");
+    std::unique_ptr<goal_mech_s> m(new goal_mech_s);
+    m->d = d;
+    m->st = st;
+    m->model = mats[0].model;
+    m->stab = stabilization ? 1 : 0;
+    m->quad_order = quad_order;
+    m->num_sets = num_sets;
+    std::vector<goal::MatParams> p(num_sets);
+    for (int i = 0; i < num_sets; ++i) {
+      require(mats[i].model == m->model, "all element sets must use the same model");
+      require(mats[i].E > 0 && mats[i].nu > -1.0 && mats[i].nu < 0.5, "invalid E or nu");
+      if (m->model == GOAL_MODEL_J2) require(mats[i].Y > 0, "J2 needs Y > 0");
+      p[i] = derive(mats[i]);
+    }
+    require(m->model == st->model, "states were created for another model");
+    m->mats.upload(p.data(), num_sets);
+    m->status.alloc(1);
+    HIPCHK(hipMemsetAsync(m->status.p, 0, sizeof(int), stream()));
+    m->elem_value.alloc(d->mesh.num_elems);
+    sync();
+    *out = m.release();
+    return GOAL_OK;
+  });
+}
+
+int goal_mech_destroy(goal_mech m) {
+  return guard([&]() -> int {
+    if (m) {
+      sync();
+      delete m;
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_mech_set_option(goal_mech m, const char* key, int64_t value) {
+  return guard([&]() -> int {
+    require(m && key, "bad arguments");
+    std::string k(key);
+    if (k == "blocks") {
+      require(value >= 0 && value < (1 << 30), "blocks out of range");
+      m->blocks = (int)value;
+    } else if (k == "xcd_remap") {
+      m->xcd_remap = value ? 1 : 0;
+    } else if (k == "check") {
+      m->check = value ? 1 : 0;
+    } else {
+      return set_error(GOAL_ERR_INVALID, "unknown option: %s", key);
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_mech_status(goal_mech m, int* flags) {
+  return guard([&]() -> int {
+    require(m && flags, "bad arguments");
+    HIPCHK(hipMemcpyAsync(flags, m->status.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    sync();
+    HIPCHK(hipMemsetAsync(m->status.p, 0, sizeof(int), stream()));
+    return GOAL_OK;
+  });
+}
+
+int goal_asm_residual(goal_mech m, goal_solinfo s, int save_states) {
+  return guard([&]() -> int {
+    require(m && s && s->d == m->d, "goal_asm_residual: bad arguments");
+    goal::AsmArgs a = asm_args(m);
+    a.R = s->R.p;
+    a.save_state = save_states ? 1 : 0;
+    HIPCHK(goal::launch_assembly(goal::ASM_RESIDUAL, m->d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
+    return finish_assembly(m);
+  });
+}
+
+int goal_asm_jacobian(goal_mech m, goal_solinfo s, int adjoint, int save_states) {
+  return guard([&]() -> int {
+    require(m && s && s->d == m->d, "goal_asm_jacobian: bad arguments");
+    goal::AsmArgs a = asm_args(m);
+    a.R = s->R.p;
+    a.vals = s->vals.p;
+    a.save_state = save_states ? 1 : 0;
+    HIPCHK(goal::launch_assembly(adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN, m->d->mesh.order, m->quad_order,
+                                 a, asm_blocks(m), stream()));
+    return finish_assembly(m);
+  });
+}
+
+int goal_asm_localize(goal_mech m, goal_solinfo s, const double* z_diff, const double* z_coarse) {
+  return guard([&]() -> int {
+    require(m && s && s->d == m->d && z_diff && z_coarse, "goal_asm_localize: bad arguments");
+    goal::AsmArgs a = asm_args(m);
+    a.R = s->R.p;
+    a.save_state = 0;
+    a.z_diff = z_diff;
+    a.z_coarse = z_coarse;
+    HIPCHK(goal::launch_assembly(goal::ASM_LOCALIZE, m->d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
+    return finish_assembly(m);
+  });
+}
+
+int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* value, double* elem_values) {
+  return guard([&]() -> int {
+    require(m && value, "goal_asm_qoi: bad arguments");
+    require(kind == GOAL_QOI_AVG_DISP || kind == GOAL_QOI_AVG_VM, "unknown QoI kind");
+    require(!s || s->d == m->d, "solinfo belongs to another disc");
+    goal::QoiArgs q{};
+    goal_disc d = m->d;
+    q.num_elems = d->mesh.num_elems;
+    q.conn = d->conn.p;
+    q.coords = d->coords.p;
+    q.x = d->x.p;
+    q.elem_set = d->elem_set.p;
+    q.mats = m->mats.p;
+    q.Fp_old = m->st->Fp_old.p;
+    q.eqps_old = m->st->eqps_old.p;
+    q.model = m->model;
+    q.kind = kind;
+    q.qoi_set = elem_set;
+    q.elem_value = m->elem_value.p;
+    q.dMdu = s ? s->dMdu.p : nullptr;
+    q.status = m->status.p;
+    HIPCHK(goal::launch_qoi(d->mesh.order, m->quad_order, q, stream()));
+    HIPCHK(goal::launch_reduce(0, m->elem_value.p, nullptr, d->mesh.num_elems, scratch(), result_slot(1), stream()));
+    HIPCHK(hipMemcpyAsync(value, result_slot(1), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    sync();
+    if (elem_values) m->elem_value.download(elem_values, d->mesh.num_elems);
+    int st = finish_assembly(m);
+    return st;
+  });
+}
+
+// ------------------------------------------------------------- Dirichlet
+int goal_eval(const char* expr, double x, double y, double z, double t, double* out) {
+  return guard([&]() -> int {
+    require(expr && out, "goal_eval: bad arguments");
+    *out = goal::eval(expr, x, y, z, t);
+    return GOAL_OK;
+  });
+}
+
+static int dbc_apply_dev(goal_disc d, goal_solinfo s, int comp, const int32_t* nodes_dev, int64_t n,
+                         const std::vector<double>& values, int jacobian) {
+  d->dbc_values.upload(values.data(), n);
+  HIPCHK(goal::launch_dbc(nodes_dev, d->dbc_values.p, n, comp, d->x.p, s->R.p, jacobian ? s->dMdu.p : nullptr,
+                          d->row_ptr.p, d->cols.p, jacobian ? s->vals.p : nullptr, stream()));
+  sync();
+  return GOAL_OK;
+}
+
+int goal_dbc_apply(goal_disc d, goal_solinfo s, int comp, const char* node_set, const char* expr, double t,
+                   int jacobian) {
+  return guard([&]() -> int {
+    require(d && s && s->d == d && node_set && expr, "goal_dbc_apply: bad arguments");
+    require(comp >= 0 && comp < goal::kDim, "DBC component must be 0..2");
+    const DevBuf<int32_t>& nodes = dev_node_set(d, node_set);
+    const std::vector<int32_t>& hn = d->mesh.node_sets[node_set];
+    goal::Expr f(expr);
+    std::vector<double> vals(hn.size());
+    for (size_t k = 0; k < hn.size(); ++k) {
+      const double* X = &d->mesh.coords[3 * (int64_t)hn[k]];
+      vals[k] = f(X[0], X[1], X[2], t);
+    }
+    return dbc_apply_dev(d, s, comp, nodes.p, (int64_t)hn.size(), vals, jacobian);
+  });
+}
+
+int goal_dbc_apply_values(goal_disc d, goal_solinfo s, int comp, int64_t n, const int32_t* nodes,
+                          const double* values, int jacobian) {
+  return guard([&]() -> int {
+    require(d && s && s->d == d && n >= 0 && (n == 0 || (nodes && values)), "goal_dbc_apply_values: bad arguments");
+    require(comp >= 0 && comp < goal::kDim, "DBC component must be 0..2");
+    for (int64_t k = 0; k < n; ++k) require(nodes[k] >= 0 && nodes[k] < d->mesh.num_nodes, "DBC node out of range");
+    DevBuf<int32_t> dn;
+    dn.upload(nodes, n);
+    std::vector<double> v(values, values + n);
+    return dbc_apply_dev(d, s, comp, dn.p, n, v, jacobian);
+  });
+}
+
+// ------------------------------------------------------------ nested DWR
+int goal_nested_create(goal_disc base, goal_nested* out) {
+  return guard([&]() -> int {
+    require(base && out, "goal_nested_create: bad arguments");
+    require(base->mesh.order == 1, "nested refinement needs a linear base mesh");
+    goal::Refinement r = goal::refine_uniform(base->mesh);
+    std::unique_ptr<goal_nested_s> n(new goal_nested_s);
+    n->base = base;
+    n->num_old = r.num_old_vertices;
+    n->num_new = (int64_t)r.edge_parents.size() / 2;
+    n->parent_host = r.parent;
+    n->edge_parents_host = r.edge_parents;
+    n->parent.upload(r.parent.data(), (int64_t)r.parent.size());
+    n->edge_parents.upload(r.edge_parents.data(), (int64_t)r.edge_parents.size());
+    n->fine = new_disc(std::move(r.fine));
+    *out = n.release();
+    return GOAL_OK;
+  });
+}
+
+int goal_nested_destroy(goal_nested n) {
+  return guard([&]() -> int {
+    if (n) {
+      sync();
+      delete n->fine;
+      delete n;
+    }
+    return GOAL_OK;
+  });
+}
+
+goal_disc goal_nested_disc(goal_nested n) { return n ? n->fine : nullptr; }
+
+int goal_nested_get_maps(goal_nested n, int32_t* parent, int32_t* edge_parents, int64_t* num_new) {
+  return guard([&]() -> int {
+    require(n != nullptr, "null nested");
+    if (parent) std::copy(n->parent_host.begin(), n->parent_host.end(), parent);
+    if (edge_parents) std::copy(n->edge_parents_host.begin(), n->edge_parents_host.end(), edge_parents);
+    if (num_new) *num_new = n->num_new;
+    return GOAL_OK;
+  });
+}
+
+int goal_nested_transfer_soln(goal_nested n) {
+  return guard([&]() -> int {
+    require(n != nullptr, "null nested");
+    HIPCHK(goal::launch_copy(n->fine->x.p, n->base->x.p, 4 * n->num_old, stream()));
+    HIPCHK(goal::launch_set_coarse(n->fine->x.p, n->edge_parents.p, n->num_old, n->num_new, stream()));
+    return GOAL_OK;
+  });
+}
+
+int goal_dwr_set_coarse(goal_nested n, double* field) {
+  return guard([&]() -> int {
+    require(n && field, "bad arguments");
+    HIPCHK(goal::launch_set_coarse(field, n->edge_parents.p, n->num_old, n->num_new, stream()));
+    return GOAL_OK;
+  });
+}
+
+int goal_dwr_subtract(const double* fine, const double* coarse, double* out, int64_t n) {
+  return guard([&]() -> int {
+    require(fine && coarse && out && n >= 0, "bad arguments");
+    HIPCHK(goal::launch_subtract(fine, coarse, out, n, stream()));
+    return GOAL_OK;
+  });
+}
+
+int goal_dwr_compute_error(goal_disc d, const double* e, double* eta) {
+  return guard([&]() -> int {
+    require(d && e && eta, "bad arguments");
+    HIPCHK(goal::launch_compute_error(d->mesh.order, d->conn.p, d->mesh.num_elems, e, eta, stream()));
+    return GOAL_OK;
+  });
+}
+
+int goal_dwr_sum_contribs(goal_disc d, const double* e, double* bound) {
+  return guard([&]() -> int {
+    require(d && e && bound, "bad arguments");
+    HIPCHK(goal::launch_sum_contribs(d->vertex_nodes.p, d->vertex_nodes.n, e, scratch(), result_slot(2), stream()));
+    HIPCHK(hipMemcpyAsync(bound, result_slot(2), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    sync();
+    return GOAL_OK;
+  });
+}
+
+int goal_dwr_set_error(goal_nested n, const double* eta_child, double* eta_base) {
+  return guard([&]() -> int {
+    require(n && eta_child && eta_base, "bad arguments");
+    HIPCHK(goal::launch_set_error(eta_child, n->base->mesh.num_elems, eta_base, stream()));
+    return GOAL_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// Launch interface between the host library (goal_capi.cpp) and the CDNA4
+// kernels.  Everything here is plain data: device pointers and sizes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace goal {
+
+enum Model : int { MODEL_ELASTIC = 0, MODEL_NEOHOOKEAN = 1, MODEL_J2 = 2 };
+
+// Per element-set material constants, derived on the host the way the
+// reference evaluators derive them in set_elem_set():
+//   mu = E/(2(1+nu)), kappa = E/(3(1-2nu)), lambda = E nu/((1+nu)(1-2nu))
+//   (src/goal_neohookean.cpp:44-52, src/goal_J2.cpp:54-64,
+//    src/goal_presidual.cpp:43-51, src/goal_stabilization.cpp:45-52)
+struct MatParams {
+  double mu, kappa, lambda, K, Y, c0;
+};
+
+// What a fused element pass computes.
+enum AsmKind : int {
+  ASM_RESIDUAL = 0,  // ST chain: R += r_e                        (Primal::compute_resid)
+  ASM_JACOBIAN = 1,  // FADT chain: R += r_e, dRdu += k_e         (Primal::compute_jacob)
+  ASM_ADJOINT = 2,   // FADT chain, transposed scatter: dRdu^T    (NestedAdjoint::compute_adjoint)
+  ASM_LOCALIZE = 3,  // ST chain, weights z_diff*N_a / z_coarse*N_a (NestedAdjoint::localize)
+};
+
+struct AsmArgs {
+  int64_t num_elems;
+  const int32_t* conn;      // [ne][npe]
+  const double* coords;     // [num_nodes][3]
+  const double* x;          // [num_dofs] u,p interleaved per node
+  const int32_t* elem_set;  // [ne]
+  const MatParams* mats;    // [num_sets]
+  const int64_t* row_ptr;   // [num_dofs + 1]
+  const uint16_t* pos;      // [ne][npe][npe]
+  double* R;                // [num_dofs] or null
+  double* vals;             // [nnz] or null
+  // per quadrature point history, structure of arrays: comp * (ne*nq) + e*nq + q
+  const double* Fp_old;
+  const double* eqps_old;
+  double* Fp;
+  double* eqps;
+  double* sigma;
+  // DWR weights (ASM_LOCALIZE): nodal fields laid out like x
+  const double* z_diff;
+  const double* z_coarse;
+  int model;
+  int stabilization;
+  int save_state;
+  int xcd_remap;            // 1: give each XCD a contiguous slab of elements
+  int* status;              // bit 0: J2 return map failed
+};
+
+// QoI selection (src/goal_mechanics.cpp:148-167)
+enum QoiKind : int { QOI_AVG_DISP = 0, QOI_AVG_VM = 1 };
+
+struct QoiArgs {
+  int64_t num_elems;
+  const int32_t* conn;
+  const double* coords;
+  const double* x;
+  const int32_t* elem_set;
+  const MatParams* mats;
+  const double* Fp_old;
+  const double* eqps_old;
+  int model;
+  int kind;
+  int qoi_set;              // element set integrated over (AvgVM), -1: all
+  double* elem_value;       // [ne] per-element QoI contribution
+  double* dMdu;             // [num_dofs] or null: derivative scatter
+  int* status;
+};
+
+// Launchers (goal_assembly.hip).  order: 1 or 2; quad_order: 1 or 2.
+hipError_t launch_assembly(int kind, int order, int quad_order, const AsmArgs& a, int blocks,
+                           hipStream_t s);
+hipError_t launch_qoi(int order, int quad_order, const QoiArgs& a, hipStream_t s);
+
+// Utility kernels (goal_util.hip)
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+hipError_t launch_axpy(double* y, const double* x, double a, int64_t n, hipStream_t s);
+hipError_t launch_expand_csr(const int64_t* node_adj_ptr, const int32_t* node_adj, int64_t num_nodes,
+                             int64_t* row_ptr, int32_t* cols, hipStream_t s);
+// Deterministic two-pass reductions into out[0] (device).  scratch holds at
+// least kReduceBlocks doubles.  op: 0 sum x, 1 sum x^2, 2 sum x*y, 3 sum |x|.
+constexpr int kReduceBlocks = 512;
+hipError_t launch_reduce(int op, const double* x, const double* y, int64_t n, double* scratch, double* out,
+                         hipStream_t s);
+// Dirichlet rows (src/goal_dbcs.cpp:39-99)
+hipError_t launch_dbc(const int32_t* nodes, const double* values, int64_t n, int comp, const double* x,
+                      double* R, double* dMdu, const int64_t* row_ptr, const int32_t* cols,
+                      double* vals, hipStream_t s);
+
+// DWR (goal_dwr.hip)
+hipError_t launch_set_coarse(double* z, const int32_t* edge_parents, int64_t num_old, int64_t num_new,
+                             hipStream_t s);
+hipError_t launch_subtract(const double* a, const double* b, double* out, int64_t n, hipStream_t s);
+hipError_t launch_compute_error(int order, const int32_t* conn, int64_t ne, const double* e,
+                                double* eta, hipStream_t s);
+hipError_t launch_sum_contribs(const int32_t* vertex_nodes, int64_t nv, const double* e, double* scratch,
+                               double* out, hipStream_t s);
+hipError_t launch_set_error(const double* eta_child, int64_t num_base, double* eta_base, hipStream_t s);
+
+}  // namespace goal

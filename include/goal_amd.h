@@ -1,0 +1,224 @@
+/*
+ * goal_amd: MI355X (gfx950) element assembly and nested-adjoint DWR error
+ * estimation for goal's mixed u-p Total Lagrangian tetrahedra.
+ *
+ * C ABI: plain pointers, sizes and opaque handles.  Every entry point returns
+ * a status (GOAL_OK = 0) and never aborts; goal_last_error() describes the
+ * last failure on the calling thread.  Handles own device memory; pointers
+ * named *_dev are device pointers, all others are host pointers.  Work is
+ * queued on one HIP stream per process; calls that hand data back to the host
+ * synchronise it.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * bgranzow/goal).
+ */
+#ifndef GOAL_AMD_H
+#define GOAL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GOAL_OK = 0,
+  GOAL_ERR_INVALID = 1,       /* bad argument / input (Teuchos validation)        */
+  GOAL_ERR_HIP = 2,           /* HIP runtime failure                              */
+  GOAL_ERR_J2_RETURN_MAP = 3, /* fail("J2: return mapping failed") src/goal_J2.cpp:119-120 */
+  GOAL_ERR_NO_DEVICE = 4,     /* no HIP device visible                            */
+  GOAL_ERR_ALLOC = 5          /* device allocation failed                         */
+};
+
+enum { GOAL_MODEL_ELASTIC = 0, GOAL_MODEL_NEOHOOKEAN = 1, GOAL_MODEL_J2 = 2 };
+enum { GOAL_QOI_AVG_DISP = 0, GOAL_QOI_AVG_VM = 1 };
+
+/* Message of the last failure on this thread (goal::fail, src/goal_control.cpp:91-99). */
+const char* goal_last_error(void);
+/* Library version string. */
+const char* goal_version(void);
+
+/* ---------------------------------------------------------------- device */
+/* Device memory and stream helpers for callers without their own runtime. */
+int goal_device_alloc(int64_t bytes, void** ptr_dev);
+int goal_device_free(void* ptr_dev);
+int goal_memcpy_h2d(void* dst_dev, const void* src, int64_t bytes);
+int goal_memcpy_d2h(void* dst, const void* src_dev, int64_t bytes);
+int goal_sync(void);
+/* dst[i] = src[i] for i < n with the library's streaming copy kernel (async). */
+int goal_device_copy(double* dst_dev, const double* src_dev, int64_t n);
+/* HIP events on the library stream: record into slot 0..255, elapsed a->b in ms. */
+int goal_event_record(int slot);
+int goal_event_elapsed(int slot_a, int slot_b, float* ms);
+/* Device name and compute-unit count. */
+int goal_device_info(char* name, int name_len, int* num_cus);
+
+/* ------------------------------------------------------------ host mesh */
+/* Host-only mesh layer (no device needed).  Call with NULL arrays to get
+ * sizes[0..1] = num_nodes, num_elems, then again with arrays of that size. */
+int goal_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, int order, double jitter, uint64_t seed,
+                  int64_t* sizes, double* coords, int32_t* conn);
+/* Host CSR graph of a mesh (the same builder goal_disc_create uses).  Call
+ * with NULL row_ptr/cols to get *nnz, then with row_ptr [4*num_nodes+1] and
+ * cols [nnz]. */
+int goal_mesh_graph(int order, int64_t num_nodes, int64_t num_elems, const int32_t* conn, int64_t* nnz,
+                    int64_t* row_ptr, int32_t* cols);
+/* Host uniform refinement of a linear mesh (see goal_nested_create).  Call
+ * with NULL outputs to get sizes[0..2] = fine nodes, fine elems, new vertices. */
+int goal_mesh_refine(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
+                     int64_t* sizes, double* fine_coords, int32_t* fine_conn, int32_t* parent,
+                     int32_t* edge_parents);
+
+/* -------------------------------------------------------- discretization */
+typedef struct goal_disc_s* goal_disc;
+
+/* Disc(ParameterList) + Disc::build_data: mesh, node-major DOF numbering
+ * lid = 4*node + eq and the CSR graph (src/goal_disc.cpp:111-120, 195-332).
+ * order 1: conn is [num_elems][4]; order 2: [num_elems][10] (4 vertices, then
+ * edges 01,12,20,03,13,23).  elem_set may be NULL (one element set). */
+int goal_disc_create(int order, int64_t num_nodes, const double* coords, int64_t num_elems,
+                     const int32_t* conn, const int32_t* elem_set, goal_disc* out);
+/* Synthetic stand-in for loadMdsMesh + read_sets (src/goal_disc.cpp:91-99):
+ * Kuhn box of nx*ny*nz hexahedra split into 6 tets each, interior vertices
+ * jittered by +-jitter*h (seeded), node sets xmin..zmax on the closed faces. */
+int goal_disc_create_box(int nx, int ny, int nz, double lx, double ly, double lz, int order, double jitter,
+                         uint64_t seed, goal_disc* out);
+/* destroy_disc (src/goal_disc.cpp:452-454) */
+int goal_disc_destroy(goal_disc d);
+/* out[0..8] = num_nodes, num_elems, num_dofs, nnz, nodes_per_elem, num_vertices, order,
+ *             num_elem_sets, max_node_degree */
+int goal_disc_sizes(goal_disc d, int64_t* out);
+int goal_disc_get_mesh(goal_disc d, double* coords, int32_t* conn, int32_t* elem_set);
+/* Ghost graph after fillComplete: rows in lid order, sorted columns
+ * (Disc::compute_graphs, src/goal_disc.cpp:307-332). row_ptr [num_dofs+1]. */
+int goal_disc_get_graph(goal_disc d, int64_t* row_ptr, int32_t* cols);
+/* Disc::get_lids for every element, [num_elems][4*nodes_per_elem] (src/goal_disc.cpp:214-222) */
+int goal_disc_get_lids(goal_disc d, int32_t* lids);
+/* Node sets (Disc::get_nodes, src/goal_disc.cpp:179-182, compute_node_sets :378-396) */
+int goal_disc_add_node_set(goal_disc d, const char* name, int64_t n, const int32_t* nodes);
+int goal_disc_node_set_size(goal_disc d, const char* name, int64_t* n);
+int goal_disc_get_node_set(goal_disc d, const char* name, int32_t* nodes);
+/* Element sets (Disc::get_elems / compute_elem_sets, src/goal_disc.cpp:169-172, 347-359) */
+int goal_disc_set_elem_sets(goal_disc d, int num_sets, const int32_t* elem_set);
+/* The u,p fields ("u", "p" created in src/goal_mechanics.cpp:71-85): x[4*node+eq]. */
+int goal_disc_set_soln(goal_disc d, const double* x);
+int goal_disc_get_soln(goal_disc d, double* x);
+/* Disc::add_soln: u,p += du (src/goal_disc.cpp:398-422) */
+int goal_disc_add_soln(goal_disc d, const double* du);
+int goal_disc_add_soln_dev(goal_disc d, const double* du_dev);
+double* goal_disc_soln_dev(goal_disc d);
+
+/* ---------------------------------------------------------------- states */
+typedef struct goal_states_s* goal_states;
+/* Mechanics::make_states (src/goal_mechanics.cpp:87-95) with one history
+ * point per quadrature point: sigma (all models), eqps and Fp with _old
+ * copies (J2).  Initial values: sigma 0, eqps 0, Fp = I (src/goal_states.cpp:99-128). */
+int goal_states_create(goal_disc d, int model, int quad_order, goal_states* out);
+int goal_states_destroy(goal_states s);
+/* States::update: X_old := X (src/goal_states.cpp:130-141) */
+int goal_states_update(goal_states s);
+/* name in {"sigma","eqps","Fp","eqps_old","Fp_old"}; layout [comp][num_elems*num_qp] */
+int goal_states_get(goal_states s, const char* name, double* out);
+int goal_states_set(goal_states s, const char* name, const double* in);
+
+/* --------------------------------------------------------------- sol info */
+typedef struct goal_solinfo_s* goal_solinfo;
+/* SolInfo: R, dMdu (num_dofs) and dRdu values on the disc graph (src/goal_sol_info.cpp:6-22) */
+int goal_solinfo_create(goal_disc d, goal_solinfo* out);
+int goal_solinfo_destroy(goal_solinfo s);
+/* zero_R / zero_dMdu / zero_dRdu / zero_all (src/goal_sol_info.cpp:51-70) */
+int goal_solinfo_zero_R(goal_solinfo s);
+int goal_solinfo_zero_dMdu(goal_solinfo s);
+int goal_solinfo_zero_dRdu(goal_solinfo s);
+int goal_solinfo_zero_all(goal_solinfo s);
+/* download; any pointer may be NULL */
+int goal_solinfo_get(goal_solinfo s, double* R, double* dMdu, double* dRdu_vals);
+int goal_solinfo_set(goal_solinfo s, const double* R, const double* dMdu, const double* dRdu_vals);
+/* device pointers of R, dMdu, dRdu values */
+int goal_solinfo_dev(goal_solinfo s, double** R_dev, double** dMdu_dev, double** vals_dev);
+/* R->norm2() (src/goal_primal.cpp:130) */
+int goal_solinfo_norm_R(goal_solinfo s, double* norm);
+/* -(R . z) (src/goal_nested_adjoint.cpp:213) with z a device vector */
+int goal_solinfo_dot_R(goal_solinfo s, const double* z_dev, double* dot);
+
+/* ------------------------------------------------------------- mechanics */
+typedef struct goal_material {
+  int model;            /* GOAL_MODEL_*                                   */
+  double E, nu, K, Y;   /* K, Y: J2 hardening modulus and yield stress    */
+  double c0;            /* stabilization constant                          */
+} goal_material;
+
+typedef struct goal_mech_s* goal_mech;
+/* Mechanics(ParameterList, Disc*) (src/goal_mechanics.cpp:41-63): one
+ * material per element set (materials sublist), stabilization on/off,
+ * quadrature order (1: the reference's one point, 2: four points).  All
+ * element sets must use the same model. */
+int goal_mech_create(goal_disc d, goal_states st, int num_sets, const goal_material* mats, int stabilization,
+                     int quad_order, goal_mech* out);
+int goal_mech_destroy(goal_mech m);
+/* Options: "blocks" (workgroups per launch, 0 = auto), "xcd_remap" (0/1),
+ * "check" (1: synchronise after each assembly and report J2 failures). */
+int goal_mech_set_option(goal_mech m, const char* key, int64_t value);
+/* Accumulated kernel status bits since the last call (1: J2 return map failed). */
+int goal_mech_status(goal_mech m, int* flags);
+
+/* goal::assemble over the chain of Mechanics::build_resid<ST> with PRIMAL
+ * scatter: R += r_e (src/goal_assembly.cpp:65-88, src/goal_primal.cpp:75-90).
+ * save_states: the `save` flag of build_resid (src/goal_primal.cpp:49-50). */
+int goal_asm_residual(goal_mech m, goal_solinfo s, int save_states);
+/* goal::assemble over build_resid<FADT>: R += r_e and dRdu += k_e
+ * (Displacement/Pressure<FADT>::scatter_primal, src/goal_displacement.cpp:177-194,
+ * src/goal_pressure.cpp:170-185); adjoint != 0 scatters the transpose
+ * (scatter_adjoint, src/goal_displacement.cpp:196-214, src/goal_pressure.cpp:187-203). */
+int goal_asm_jacobian(goal_mech m, goal_solinfo s, int adjoint, int save_states);
+/* Functional::compute / QoI<ST>, QoI<FADT> (src/goal_functional.cpp:62-70,
+ * src/goal_qoi.cpp:20-81, src/goal_avg_disp.cpp:16-21, src/goal_avg_vm.cpp:32-61).
+ * kind GOAL_QOI_*; elem_set: AvgVM element set (-1 = all); value: J(u);
+ * elem_values (host, may be NULL): per-element contributions; if s != NULL
+ * the derivative dJ/du is added into s->dMdu. */
+int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* value, double* elem_values);
+/* NestedAdjoint::localize's assembly (src/goal_nested_adjoint.cpp:217-234):
+ * residual of build_error (src/goal_mechanics.cpp:169-218) with weights
+ * z_diff*N_a (uw, pw) and z_coarse*N_a (pwc), R += r_e. */
+int goal_asm_localize(goal_mech m, goal_solinfo s, const double* z_diff_dev, const double* z_coarse_dev);
+
+/* ------------------------------------------------------------ Dirichlet */
+/* goal::eval (src/goal_control.cpp:107-123) */
+int goal_eval(const char* expr, double x, double y, double z, double t, double* out);
+/* One entry [comp, node set, expr] of set_resid_dbcs (jacobian = 0) or
+ * set_jac_dbcs (jacobian = 1) (src/goal_dbcs.cpp:39-99). */
+int goal_dbc_apply(goal_disc d, goal_solinfo s, int comp, const char* node_set, const char* expr, double t,
+                   int jacobian);
+/* Same with explicit node ids and prescribed values. */
+int goal_dbc_apply_values(goal_disc d, goal_solinfo s, int comp, int64_t n, const int32_t* nodes,
+                          const double* values, int jacobian);
+
+/* ------------------------------------------------------------ nested DWR */
+typedef struct goal_nested_s* goal_nested;
+/* Nested(Disc*, UNIFORM) (src/goal_nested.cpp:13-28, 144-154): uniform
+ * 1:8 refinement of a linear base mesh; the refined disc carries u,p
+ * transferred from the base (new vertex = mean of its edge's vertices). */
+int goal_nested_create(goal_disc base, goal_nested* out);
+int goal_nested_destroy(goal_nested n);
+/* The refined discretization, owned by n. */
+goal_disc goal_nested_disc(goal_nested n);
+/* parent [num_fine_elems]; edge_parents [num_new_vertices][2] (may be NULL) */
+int goal_nested_get_maps(goal_nested n, int32_t* parent, int32_t* edge_parents, int64_t* num_new_vertices);
+/* Copy base u,p to the old vertices of the nested solution and interpolate new ones. */
+int goal_nested_transfer_soln(goal_nested n);
+/* Nested::set_coarse on a nested nodal field, in place (src/goal_nested.cpp:369-393) */
+int goal_dwr_set_coarse(goal_nested n, double* field_dev);
+/* NestedAdjoint::subtract (src/goal_nested_adjoint.cpp:182-197): out = fine - coarse */
+int goal_dwr_subtract(const double* fine_dev, const double* coarse_dev, double* out_dev, int64_t n);
+/* compute_error (src/goal_error.cpp:7-35): eta[e] = |sum_d e_u,d + e_p| at the centroid */
+int goal_dwr_compute_error(goal_disc d, const double* e_dev, double* eta_dev);
+/* sum_contribs (src/goal_error.cpp:37-56) */
+int goal_dwr_sum_contribs(goal_disc d, const double* e_dev, double* bound);
+/* Nested::set_error (src/goal_nested.cpp:395-412): eta_base[E] = sum of its children */
+int goal_dwr_set_error(goal_nested n, const double* eta_child_dev, double* eta_base_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOAL_AMD_H */

@@ -1,0 +1,201 @@
+"""HIP path (through the C ABI) against the oracle on the same seeded inputs.
+
+Bar: CSR structure and index maps bit-exact; R, dRdu, dMdu, states, QoI and
+indicators within RTOL = 1e-11 normwise relative (FP64 arithmetic on both
+sides, different operation order and atomic accumulation order)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import MATS, RTOL, mat_row, oracle_mesh_from, random_history, random_state, rel
+
+pytestmark = pytest.mark.gpu
+
+g = pytest.importorskip("goal_amd")
+
+CASES = [
+    # (order, quad_order, model)
+    (1, 1, 1), (1, 1, 2), (1, 1, 0),
+    (1, 2, 1), (1, 2, 2),
+    (2, 2, 1), (2, 2, 2), (2, 2, 0),
+]
+
+
+def make(order, quad, model, n=3, seed=11, jitter=0.1, history=True):
+    disc = g.Disc.box(n, n, n, order=order, jitter=jitter, seed=seed)
+    m = MATS[model]
+    st = g.States(disc, model, quad)
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(**m)], stabilization=True, quad_order=quad)
+    x = random_state(disc.num_nodes, seed)
+    disc.set_soln(x)
+    om = oracle_mesh_from(disc)
+    ost = O.States(om, model, quad)
+    if model == 2 and history:
+        Fp, eq = random_history(disc.num_elems * st.nq, seed + 1)
+        st.set("Fp_old", Fp)
+        st.set("eqps_old", eq)
+        st.set("Fp", Fp)
+        st.set("eqps", eq)
+        ost.Fp_old[:] = Fp
+        ost.eqps_old[:] = eq
+        ost.Fp[:] = Fp
+        ost.eqps[:] = eq
+    return disc, st, mech, x, om, ost, [mat_row(m)]
+
+
+@pytest.mark.parametrize("order,quad,model", CASES)
+def test_graph_bit_exact(order, quad, model):
+    disc = g.Disc.box(3, 2, 2, order=order, jitter=0.1, seed=3)
+    rp, cols = disc.graph()
+    om = oracle_mesh_from(disc)
+    assert np.array_equal(rp, om.row_ptr)
+    assert np.array_equal(cols, om.cols)
+    coords, conn, _ = disc.mesh()
+    lids = disc.lids()
+    ref = (4 * conn[:, :, None] + np.arange(4)[None, None, :]).reshape(conn.shape[0], -1)
+    assert np.array_equal(lids, ref)
+
+
+@pytest.mark.parametrize("order,quad,model", CASES)
+def test_residual(order, quad, model):
+    disc, st, mech, x, om, ost, mats = make(order, quad, model)
+    si = g.SolInfo(disc)
+    si.zero_R()
+    mech.residual(si, save_states=True)
+    Rg = si.R()
+    Ro, _ = O.assemble(O.K_RESIDUAL, om, model, mats, x, ost, quad_order=quad)
+    assert rel(Rg, Ro) < RTOL
+    assert rel(st.get("sigma"), ost.sigma) < RTOL
+    if model == 2:
+        assert rel(st.get("eqps"), ost.eqps) < RTOL
+        assert rel(st.get("Fp"), ost.Fp) < RTOL
+
+
+@pytest.mark.parametrize("order,quad,model", CASES)
+def test_jacobian(order, quad, model):
+    disc, st, mech, x, om, ost, mats = make(order, quad, model)
+    si = g.SolInfo(disc)
+    si.zero_all()
+    mech.jacobian(si, adjoint=False, save_states=True)
+    Rg, Vg = si.R(), si.vals()
+    Ro, Vo = O.assemble(O.K_JACOBIAN, om, model, mats, x, ost, quad_order=quad)
+    assert rel(Rg, Ro) < RTOL
+    assert rel(Vg, Vo) < RTOL
+    if model == 2:
+        assert rel(st.get("Fp"), ost.Fp) < RTOL
+
+
+@pytest.mark.parametrize("order,quad,model", [(1, 1, 1), (1, 1, 2), (2, 2, 1)])
+def test_jacobian_adjoint(order, quad, model):
+    disc, st, mech, x, om, ost, mats = make(order, quad, model)
+    si = g.SolInfo(disc)
+    si.zero_all()
+    mech.jacobian(si, adjoint=True, save_states=False)
+    Ro, Vo = O.assemble(O.K_ADJOINT, om, model, mats, x, ost, quad_order=quad, save=False)
+    assert rel(si.R(), Ro) < RTOL
+    assert rel(si.vals(), Vo) < RTOL
+
+
+@pytest.mark.parametrize("order,quad,model", [(1, 1, 1), (2, 2, 1), (1, 1, 2), (2, 2, 2)])
+def test_qoi(order, quad, model):
+    disc, st, mech, x, om, ost, mats = make(order, quad, model)
+    si = g.SolInfo(disc)
+    for kind in (g.QOI_AVG_DISP, g.QOI_AVG_VM):
+        si.zero_dMdu()
+        val, ev = mech.qoi(kind, si, elem_set=-1, elem_values=True)
+        vo, evo, dmo = O.qoi(kind, om, model, mats, x, ost, quad_order=quad, derivative=True)
+        assert abs(val - vo) <= RTOL * abs(vo)
+        assert rel(ev, evo) < RTOL
+        assert rel(si.dMdu(), dmo) < RTOL
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_dirichlet_rows(order):
+    quad = 1 if order == 1 else 2
+    disc, st, mech, x, om, ost, mats = make(order, quad, 1)
+    si = g.SolInfo(disc)
+    si.zero_all()
+    mech.jacobian(si)
+    R, V = si.R(), si.vals()
+    dM = np.linspace(0.0, 1.0, disc.num_dofs)
+    si_dm = dM.copy()
+    import ctypes as C
+    g.check(g.lib.goal_solinfo_set(si.h, None, si_dm.ctypes.data_as(C.c_void_p), None))
+    for comp in range(3):
+        g.dbc_apply(disc, si, comp, "zmin", "0.0", 1.0, jacobian=True)
+    g.dbc_apply(disc, si, 2, "zmax", "0.01*t", 3.0, jacobian=True)
+    Ro, Vo, dMo = R.copy(), V.copy(), dM.copy()
+    coords = om.coords
+    zmin, zmax = disc.node_set("zmin"), disc.node_set("zmax")
+    for comp in range(3):
+        O.dbc(om, zmin, np.zeros(zmin.size), comp, x, Ro, dMo, Vo)
+    O.dbc(om, zmax, 0.01 * 3.0 * np.ones(zmax.size), 2, x, Ro, dMo, Vo)
+    assert np.array_equal(si.R(), Ro)
+    assert np.array_equal(si.vals(), Vo)
+    assert np.array_equal(si.dMdu(), dMo)
+    assert np.allclose(coords[zmax, 2], 1.0) and np.allclose(coords[zmin, 2], 0.0)
+
+
+@pytest.mark.parametrize("model", [1, 2])
+def test_localize_and_indicators(model):
+    base = g.Disc.box(2, 2, 2, order=1, jitter=0.1, seed=5)
+    nested = g.Nested(base)
+    fine = nested.disc
+    quad = 1
+    m = MATS[model]
+    st = g.States(fine, model, quad)
+    mech = g.Mechanics(fine, st, [g.MaterialSpec(**m)], quad_order=quad)
+    xb = random_state(base.num_nodes, 5)
+    base.set_soln(xb)
+    nested.transfer_soln()
+    x = fine.soln()
+    parent, ep = nested.maps()
+    # transfer: old vertices copied, new ones averaged
+    xo = np.zeros(fine.num_dofs)
+    xo[: xb.size] = xb
+    O.set_coarse(xo, ep, base.num_nodes)
+    assert np.array_equal(x, xo)
+    om = oracle_mesh_from(fine)
+    ost = O.States(om, model, quad)
+    rng = np.random.default_rng(9)
+    z_fine = rng.standard_normal(fine.num_dofs)
+    zf = g.DeviceArray.from_host(z_fine)
+    zc = g.DeviceArray.from_host(z_fine)
+    nested.set_coarse(zc)
+    zco = z_fine.copy()
+    O.set_coarse(zco, ep, base.num_nodes)
+    assert np.array_equal(zc.numpy(), zco)
+    zd = g.DeviceArray(fine.num_dofs)
+    g.dwr_subtract(zf, zc, zd)
+    zdo = z_fine - zco
+    assert np.array_equal(zd.numpy(), zdo)
+    si = g.SolInfo(fine)
+    si.zero_R()
+    mech.localize(si, zd.ptr, zc.ptr)
+    Ro, _ = O.assemble(O.K_LOCALIZE, om, model, [mat_row(m)], x, ost, quad_order=quad, save=False,
+                       z_diff=zdo, z_coarse=zco)
+    assert rel(si.R(), Ro) < RTOL
+    R_dev = si.dev()[0]
+    eta = g.DeviceArray(fine.num_elems)
+    g.dwr_compute_error(fine, R_dev, eta)
+    eta_o = O.compute_error(om, Ro)
+    assert rel(eta.numpy(), eta_o) < RTOL
+    bound = g.dwr_sum_contribs(fine, R_dev)
+    bound_o = O.sum_contribs(np.arange(fine.num_nodes), Ro)
+    assert abs(bound - bound_o) <= RTOL * abs(bound_o)
+    eb = g.DeviceArray(base.num_elems)
+    nested.set_error(eta, eb)
+    eb_o = O.set_error(parent, eta.numpy(), base.num_elems)
+    assert rel(eb.numpy(), eb_o) < 1e-15
+
+
+def test_norm_and_dot():
+    disc = g.Disc.box(2, 2, 2, order=1)
+    si = g.SolInfo(disc)
+    rng = np.random.default_rng(1)
+    R = rng.standard_normal(disc.num_dofs)
+    import ctypes as C
+    g.check(g.lib.goal_solinfo_set(si.h, R.ctypes.data_as(C.c_void_p), None, None))
+    assert abs(si.norm_R() - np.linalg.norm(R)) < 1e-13 * np.linalg.norm(R)
+    z = g.DeviceArray.from_host(rng.standard_normal(disc.num_dofs))
+    assert abs(si.dot_R(z.ptr) + R @ z.numpy()) < 1e-12 * np.abs(R).sum()
