@@ -19,7 +19,8 @@ from pathlib import Path
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-LIB_PATH = _HERE / "libgoal_amd.so"
+# GOAL_AMD_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = Path(os.environ["GOAL_AMD_LIB"]) if os.environ.get("GOAL_AMD_LIB") else _HERE / "libgoal_amd.so"
 
 OK = 0
 ERR_INVALID = 1
@@ -93,6 +94,8 @@ _sig("goal_disc_destroy", _i32, _p)
 _sig("goal_disc_sizes", _i32, _p, _p)
 _sig("goal_disc_get_mesh", _i32, _p, _p, _p, _p)
 _sig("goal_disc_get_graph", _i32, _p, _p, _p)
+_sig("goal_disc_graph_dev", _i32, _p, C.POINTER(_p), C.POINTER(_p))
+_sig("goal_disc_mesh_dev", _i32, _p, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p))
 _sig("goal_disc_get_lids", _i32, _p, _p)
 _sig("goal_disc_add_node_set", _i32, _p, C.c_char_p, _i64, _p)
 _sig("goal_disc_node_set_size", _i32, _p, C.c_char_p, _i64p)
@@ -296,6 +299,16 @@ class Disc:
         cols = np.empty(self.nnz, dtype=np.int32)
         check(lib.goal_disc_get_graph(self.h, _ptr(rp), _ptr(cols)))
         return rp, cols
+
+    def graph_dev(self):
+        rp, cols = _p(), _p()
+        check(lib.goal_disc_graph_dev(self.h, C.byref(rp), C.byref(cols)))
+        return rp.value, cols.value
+
+    def mesh_dev(self):
+        c, cn, ps = _p(), _p(), _p()
+        check(lib.goal_disc_mesh_dev(self.h, C.byref(c), C.byref(cn), C.byref(ps)))
+        return c.value, cn.value, ps.value
 
     def lids(self):
         out = np.empty((self.num_elems, 4 * self.nodes_per_elem), dtype=np.int32)

@@ -19,8 +19,12 @@ LIB = PKG / "libgoal_amd.so"
 
 ARCH = os.environ.get("GOAL_AMD_ARCH", "gfx950")
 
+# Minimum waves per SIMD requested for the element kernel (register cap).
+ELEM_WAVES = int(os.environ.get("GOAL_ELEM_WAVES", "2"))
+
 COMMON = [
     "-O3",
+    f"-DGOAL_ELEM_WAVES={ELEM_WAVES}",
     "-fPIC",
     "-std=c++17",
     "-Wall",
@@ -64,7 +68,8 @@ def _compile(src: Path, obj: Path, verbose: bool) -> None:
         print(r.stderr, file=sys.stderr)
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, out: Path | None = None) -> Path:
+    lib = Path(out) if out else LIB
     BUILD.mkdir(exist_ok=True)
     objs = []
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
@@ -76,16 +81,17 @@ def build(verbose: bool = False) -> Path:
             futs.append(ex.submit(_compile, src, obj, verbose))
         for f in futs:
             f.result()
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp)] + [str(o) for o in objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    args = [a for a in sys.argv[1:] if a != "-v"]
+    print(build(verbose="-v" in sys.argv, out=Path(args[0]) if args else None))

@@ -1,23 +1,30 @@
-// Fused element assembly for goal's mixed u-p Total Lagrangian tets.
+// Element assembly for goal's mixed u-p Total Lagrangian tets, in two passes.
 //
-// One kernel performs, per element, the whole evaluator chain that
+// Pass 1 (element_kernel) runs, per element, the evaluator chain that
 // goal::assemble() runs hook by hook (src/goal_assembly.cpp:65-88, chain order
-// src/goal_mechanics.cpp:97-146): gather of u and p, interpolation, kinematics
-// F = I + grad u, J, the constitutive model, Mixed, MResidual, PResidual,
-// Stabilization, and the scatter into the FP64 residual and CSR Jacobian.
+// src/goal_mechanics.cpp:97-146): gather of u and p, interpolation at the
+// quadrature points, kinematics F = I + grad u and J, the constitutive model,
+// Mixed, MResidual, PResidual and Stabilization.  It writes the element
+// residual r_e and, for the Jacobian, the element matrix k_e to element-major
+// buffers with coalesced stores.
 //
-// Work layout (CDNA4, wave64): a workgroup is one wavefront and processes a
-// group of EPW elements at a time (EPW = 4 for 4-node tets: 16 lanes per
-// element; EPW = 1 for 10-node tets).  Element data (connectivity, vertex
-// coordinates, DOFs, CSR row starts, node-pair positions) is staged in LDS.
-// Lanes then cooperate in four phases: shape functions per (qp, node);
-// constitutive update per (qp, direction) - each lane evaluates one column of
-// the 13x13 generalized tangent; residual rows; and Jacobian columns, where a
-// lane holds D*B for its column in registers and walks the element rows,
-// adding each entry into the CSR with a hardware FP64 atomic (4 adjacent lanes
-// hit 4 adjacent columns of one row).  Workgroups can be mapped so that each
-// XCD sweeps one contiguous slab of the element range (xcd_remap), keeping the
-// CSR rows an XCD accumulates into within its own L2.
+// Pass 2 (gather_kernel) is the scatter (Displacement/Pressure::scatter,
+// src/goal_displacement.cpp:77-86,177-214, src/goal_pressure.cpp:76-83,170-203)
+// turned around: one wavefront owns one mesh node, i.e. the 4 consecutive CSR
+// rows of that node, sums the row blocks of its incident elements in LDS and
+// writes the rows once.  No atomics, no zero fill, and the summation order is
+// fixed, so results are bitwise reproducible.
+//
+// Pass 1 work layout (wave64): a workgroup is one wavefront and processes EPW
+// elements at a time (4 for 4-node tets: 16 lanes each; 1 for 10-node tets).
+// Phases, separated by wave barriers, with all element data in LDS:
+//   A  gather connectivity, vertex coordinates, DOFs
+//   B  shape functions per (quadrature point, node)
+//   C1 quadrature-point state per quadrature point (kinematics, model, states)
+//   C2 one column of the 13x13 tangent per (quadrature point, direction)
+//   D  residual rows
+//   E  Jacobian: a lane owns one column (b,k), forms C = D B for one row
+//      group at a time and writes K(row, col) = B^T C for all rows.
 
 #include "goal_device.hpp"
 #include "goal_kernels.hpp"
@@ -27,8 +34,36 @@ namespace {
 
 using dev::Quad;
 
-template <int NN, int NQ>
+template <int MODEL>
+struct ModelOf;
+template <>
+struct ModelOf<MODEL_ELASTIC> {
+  using type = dev::Elastic;
+};
+template <>
+struct ModelOf<MODEL_NEOHOOKEAN> {
+  using type = dev::NeoHookean;
+};
+template <>
+struct ModelOf<MODEL_J2> {
+  using type = dev::J2Model;
+};
+
+// What phase C1 leaves in LDS for the tangent columns of phase C2.
+template <int MODEL>
+struct QPStore {
+  dev::QP qp;
+  dev::Stab st;
+  typename ModelOf<MODEL>::type m;
+};
+template <>
+struct QPStore<MODEL_NEOHOOKEAN> {
+  dev::NHBase b;
+};
+
+template <int NN, int NQ, int MODEL>
 struct Smem {
+  using M = typename ModelOf<MODEL>::type;
   static constexpr int EPW = (NN == 4) ? 4 : 1;
   static constexpr int ND = 4 * NN;
   double x[EPW][ND];
@@ -37,14 +72,13 @@ struct Smem {
   double wdv[EPW][NQ];
   double sig[EPW][NQ][13];
   double D[EPW][NQ][13][13];
+  QPStore<MODEL> qs[EPW][NQ];
   double zq[EPW][NQ][4];      // z_diff (u,p) at qp            (localize)
   double zgq[EPW][NQ][4][3];  // grad z_diff                   (localize)
   double zcq[EPW][NQ][4];     // z_coarse p and its gradient   (localize)
   double zd[EPW][ND];
   double zc[EPW][NN];
-  int64_t rowbase[EPW][ND];
   int32_t conn[EPW][NN];
-  uint16_t pos[EPW][NN * NN];
 };
 
 // Element-group schedule.  Plain: group = block + it * gridDim.  XCD slabs:
@@ -65,26 +99,54 @@ __device__ inline int64_t group_of(int64_t it, int64_t ngroups, int remap) {
   return g < ngroups ? g : -1;
 }
 
-__device__ inline void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+// Column col = (b, kc) of the element matrix, all rows at once:
+//   C_q = D_q B_q[:, col]  (13 values),   K[(a,i)][col] = sum_q B_q[:, (a,i)]^T C_q.
+// Quadrature points are the outer loop so each shape-function row of the LDS
+// table is read once per column; the NN*4 sums stay in registers.
+template <int NN, int NQ>
+__device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[13][13], int col, double* out) {
+  constexpr int ND = 4 * NN;
+  const int b = col >> 2, kc = col & 3;
+  double acc[NN][4];
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+    for (int i = 0; i < 4; ++i) acc[n][i] = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < NQ; ++q) {
+    const double* shb = sh[q][b];
+    const double sb0 = shb[0], sb1 = shb[1], sb2 = shb[2], sb3 = shb[3];
+    double C[13];
+#pragma unroll
+    for (int r = 0; r < 13; ++r) {
+      const double* Dr = D[q][r];
+      C[r] = (kc < 3) ? Dr[3 * kc] * sb1 + Dr[3 * kc + 1] * sb2 + Dr[3 * kc + 2] * sb3
+                      : Dr[9] * sb0 + Dr[10] * sb1 + Dr[11] * sb2 + Dr[12] * sb3;
+    }
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      const double* sa = sh[q][n];
+      const double s0 = sa[0], s1 = sa[1], s2 = sa[2], s3 = sa[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[n][i] += C[3 * i] * s1 + C[3 * i + 1] * s2 + C[3 * i + 2] * s3;
+      acc[n][3] += C[9] * s0 + C[10] * s1 + C[11] * s2 + C[12] * s3;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+    for (int i = 0; i < 4; ++i) out[(4 * n + i) * ND] = acc[n][i];
+}
 
+// Waves per SIMD the compiler must leave room for: the J2 return map needs the
+// larger register budget.
 template <int MODEL>
-struct ModelOf;
-template <>
-struct ModelOf<MODEL_ELASTIC> {
-  using type = dev::Elastic;
-};
-template <>
-struct ModelOf<MODEL_NEOHOOKEAN> {
-  using type = dev::NeoHookean;
-};
-template <>
-struct ModelOf<MODEL_J2> {
-  using type = dev::J2Model;
-};
+constexpr int elem_waves() {
+  return MODEL == MODEL_J2 ? 1 : GOAL_ELEM_WAVES;
+}
 
 template <int NN, int NQ, int MODEL, int KIND>
-__global__ __launch_bounds__(64) void assembly_kernel(AsmArgs a) {
-  using S = Smem<NN, NQ>;
+__global__ __launch_bounds__(64, elem_waves<MODEL>()) void element_kernel(AsmArgs a) {
+  using S = Smem<NN, NQ, MODEL>;
+  using M = typename S::M;
   constexpr int EPW = S::EPW, ND = S::ND, LPE = 64 / EPW;
   constexpr bool JAC = (KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT);
   constexpr bool LOC = (KIND == ASM_LOCALIZE);
@@ -102,7 +164,7 @@ __global__ __launch_bounds__(64) void assembly_kernel(AsmArgs a) {
     const int64_t e = grp * EPW + el;
     const bool valid = e < ne;
 
-    // ---- gather (Displacement/Pressure::gather, src/goal_displacement.cpp:139-156)
+    // ---- A: gather (Displacement/Pressure::gather, src/goal_displacement.cpp:139-156)
     if (valid) {
       const int32_t* ec = a.conn + e * NN;
       if (l < NN) sm.conn[el][l] = ec[l];
@@ -110,17 +172,14 @@ __global__ __launch_bounds__(64) void assembly_kernel(AsmArgs a) {
       for (int k = l; k < ND; k += LPE) {
         const int64_t dof = 4 * (int64_t)ec[k >> 2] + (k & 3);
         sm.x[el][k] = a.x[dof];
-        if (JAC) sm.rowbase[el][k] = a.row_ptr[dof];
         if (LOC) sm.zd[el][k] = a.z_diff[dof];
       }
       if (LOC)
         for (int k = l; k < NN; k += LPE) sm.zc[el][k] = a.z_coarse[4 * (int64_t)ec[k] + 3];
-      if (JAC)
-        for (int k = l; k < NN * NN; k += LPE) sm.pos[el][k] = a.pos[e * (NN * NN) + k];
     }
     __syncthreads();
 
-    // ---- shape functions at the quadrature points (u/uw/p/pw at_point)
+    // ---- B: shape functions at the quadrature points (u/uw/p/pw at_point)
     if (valid)
       for (int t = l; t < NQ * NN; t += LPE) {
         const int q = t / NN, n = t % NN;
@@ -139,16 +198,15 @@ __global__ __launch_bounds__(64) void assembly_kernel(AsmArgs a) {
       }
     __syncthreads();
 
-    // ---- constitutive update: lane (qp, c), c < 9 -> tangent column F_kL,
-    //      c == 9 -> value, p and grad-p columns, state writes
+    // ---- C1: quadrature-point state (Kinematics, model, Mixed; states)
     if (valid)
-      for (int t = l; t < NQ * 10; t += LPE) {
-        const int q = t / 10, c = t % 10;
+      for (int q = l; q < NQ; q += LPE) {
         const MatParams mp = a.mats[a.elem_set ? a.elem_set[e] : 0];
         dev::QP qp;
         for (int k = 0; k < 9; ++k) qp.H[k] = 0.0;
         qp.p = 0.0;
         qp.g[0] = qp.g[1] = qp.g[2] = 0.0;
+#pragma unroll 1
         for (int n = 0; n < NN; ++n) {
           const double N = sm.sh[el][q][n][0];
           const double pn = sm.x[el][4 * n + 3];
@@ -163,95 +221,107 @@ __global__ __launch_bounds__(64) void assembly_kernel(AsmArgs a) {
         qp.J = dev::det3(qp.F);
         dev::inv3(qp.F, qp.J, qp.Fi);
         dev::Stab st;
-        const double tau = a.stabilization ? dev::stab_tau(sm.X[el], mp) : 0.0;
-        st.init(qp, tau);
+        st.init(qp, a.stabilization ? dev::stab_tau(sm.X[el], mp) : 0.0);
         const double wdv = sm.wdv[el][q];
         const int64_t sidx = e * NQ + q;
-
-        typename ModelOf<MODEL>::type m;
+        M m;
         if constexpr (MODEL == MODEL_J2) {
           double Fpo[9];
           for (int k = 0; k < 9; ++k) Fpo[k] = a.Fp_old[k * nqtot + sidx];
           m.init(qp, mp, Fpo, a.eqps_old[sidx]);
-          if (c == 9) {
-            if (m.failed()) atomicOr(a.status, 1);
-            if (a.save_state) {
-              a.eqps[sidx] = m.alpha;
-              if (m.plastic) {
-                double Fpn[9];
-                m.new_Fp(Fpo, Fpn);
-                for (int k = 0; k < 9; ++k) a.Fp[k * nqtot + sidx] = Fpn[k];
-              }
+          if (m.failed()) atomicOr(a.status, 1);
+          if (a.save_state) {
+            a.eqps[sidx] = m.alpha;
+            if (m.plastic) {
+              double Fpn[9];
+              m.new_Fp(Fpo, Fpn);
+              for (int k = 0; k < 9; ++k) a.Fp[k * nqtot + sidx] = Fpn[k];
             }
           }
         } else {
           m.init(qp, mp);
         }
-        if (c == 9) {
-          if (a.save_state)
-            for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
-          for (int k = 0; k < 9; ++k) sm.sig[el][q][k] = m.P[k] * wdv;
-          sm.sig[el][q][9] = m.r1(qp) * wdv;
-          double s3[3];
-          st.value(s3);
-          for (int j = 0; j < 3; ++j) sm.sig[el][q][10 + j] = s3[j] * wdv;
-          if (LOC) {
-            for (int i = 0; i < 4; ++i) {
-              double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
-              for (int n = 0; n < NN; ++n) {
-                const double zn = sm.zd[el][4 * n + i];
-                z += zn * sm.sh[el][q][n][0];
-                for (int j = 0; j < 3; ++j) zg[j] += zn * sm.sh[el][q][n][1 + j];
-              }
-              sm.zq[el][q][i] = z;
-              for (int j = 0; j < 3; ++j) sm.zgq[el][q][i][j] = zg[j];
-            }
+        if (a.save_state)
+          for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
+        for (int k = 0; k < 9; ++k) sm.sig[el][q][k] = m.P[k] * wdv;
+        sm.sig[el][q][9] = m.r1(qp) * wdv;
+        double s3[3];
+        st.value(s3);
+        for (int j = 0; j < 3; ++j) sm.sig[el][q][10 + j] = s3[j] * wdv;
+        if (JAC) {
+          if constexpr (MODEL == MODEL_NEOHOOKEAN) {
+            dev::nh_base(qp, m, st, sm.qs[el][q].b);
+          } else {
+            sm.qs[el][q].qp = qp;
+            sm.qs[el][q].st = st;
+            sm.qs[el][q].m = m;
+          }
+          // p column: dsigma/dp = I (Mixed); grad-p columns: stabilization only
+          double dPp[9], dr1p;
+          if constexpr (MODEL == MODEL_ELASTIC) {
+            for (int k = 0; k < 9; ++k) dPp[k] = (k % 4 == 0) ? 1.0 : 0.0;
+            dr1p = 1.0;
+          } else {
+            for (int i = 0; i < 3; ++i)
+              for (int jj = 0; jj < 3; ++jj) dPp[3 * i + jj] = qp.J * qp.Fi[3 * jj + i];
+            dr1p = 1.0 / mp.kappa;
+          }
+          for (int r = 0; r < 9; ++r) sm.D[el][q][r][9] = dPp[r] * wdv;
+          sm.D[el][q][9][9] = dr1p * wdv;
+          for (int r = 10; r < 13; ++r) sm.D[el][q][r][9] = 0.0;
+          for (int I = 0; I < 3; ++I) {
+            double ds3[3];
+            st.dir_g(qp, I, ds3);
+            for (int r = 0; r < 10; ++r) sm.D[el][q][r][10 + I] = 0.0;
+            for (int j = 0; j < 3; ++j) sm.D[el][q][10 + j][10 + I] = ds3[j] * wdv;
+          }
+        }
+        if (LOC) {
+          for (int i = 0; i < 4; ++i) {
             double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
             for (int n = 0; n < NN; ++n) {
-              const double zn = sm.zc[el][n];
+              const double zn = sm.zd[el][4 * n + i];
               z += zn * sm.sh[el][q][n][0];
               for (int j = 0; j < 3; ++j) zg[j] += zn * sm.sh[el][q][n][1 + j];
             }
-            sm.zcq[el][q][0] = z;
-            for (int j = 0; j < 3; ++j) sm.zcq[el][q][1 + j] = zg[j];
+            sm.zq[el][q][i] = z;
+            for (int j = 0; j < 3; ++j) sm.zgq[el][q][i][j] = zg[j];
           }
-          if (JAC) {
-            // p column: dsigma = I (Mixed), dP = d(P)/dp, dr1 = dr1/dp
-            double dPp[9], dr1p;
-            if constexpr (MODEL == MODEL_ELASTIC) {
-              for (int k = 0; k < 9; ++k) dPp[k] = (k % 4 == 0) ? 1.0 : 0.0;
-              dr1p = 1.0;
-            } else {
-              for (int i = 0; i < 3; ++i)
-                for (int jj = 0; jj < 3; ++jj) dPp[3 * i + jj] = qp.J * qp.Fi[3 * jj + i];
-              dr1p = 1.0 / mp.kappa;
-            }
-            for (int r = 0; r < 9; ++r) sm.D[el][q][r][9] = dPp[r] * wdv;
-            sm.D[el][q][9][9] = dr1p * wdv;
-            for (int r = 10; r < 13; ++r) sm.D[el][q][r][9] = 0.0;
-            // grad-p columns: only the stabilization flux depends on grad p
-            for (int I = 0; I < 3; ++I) {
-              double ds3[3];
-              st.dir_g(qp, I, ds3);
-              for (int r = 0; r < 10; ++r) sm.D[el][q][r][10 + I] = 0.0;
-              for (int j = 0; j < 3; ++j) sm.D[el][q][10 + j][10 + I] = ds3[j] * wdv;
-            }
+          double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
+          for (int n = 0; n < NN; ++n) {
+            const double zn = sm.zc[el][n];
+            z += zn * sm.sh[el][q][n][0];
+            for (int j = 0; j < 3; ++j) zg[j] += zn * sm.sh[el][q][n][1 + j];
           }
-        } else if (JAC) {
-          const int kk = c / 3, LL = c % 3;
-          dev::Dir d;
-          m.dir(qp, kk, LL, d);
-          double ds3[3];
-          st.dir(qp, kk, LL, ds3);
-          for (int r = 0; r < 9; ++r) sm.D[el][q][r][c] = d.dP[r] * wdv;
-          sm.D[el][q][9][c] = d.dr1 * wdv;
-          for (int j = 0; j < 3; ++j) sm.D[el][q][10 + j][c] = ds3[j] * wdv;
+          sm.zcq[el][q][0] = z;
+          for (int j = 0; j < 3; ++j) sm.zcq[el][q][1 + j] = zg[j];
         }
       }
     __syncthreads();
 
-    // ---- residual rows (MResidual, PResidual, Stabilization + scatter)
-    if (valid && a.R)
+    // ---- C2: tangent columns d sig / d F_kL, one (qp, direction) per lane
+    if (JAC) {
+      if (valid)
+        for (int t = l; t < NQ * 9; t += LPE) {
+          const int q = t / 9, c = t % 9;
+          const double wdv = sm.wdv[el][q];
+          if constexpr (MODEL == MODEL_NEOHOOKEAN) {
+            dev::nh_column(sm.qs[el][q].b, c / 3, c % 3, wdv, &sm.D[el][q][0][c], 13);
+          } else {
+            dev::Dir d;
+            sm.qs[el][q].m.dir(sm.qs[el][q].qp, c / 3, c % 3, d);
+            double ds3[3];
+            sm.qs[el][q].st.dir(sm.qs[el][q].qp, c / 3, c % 3, ds3);
+            for (int r = 0; r < 9; ++r) sm.D[el][q][r][c] = d.dP[r] * wdv;
+            sm.D[el][q][9][c] = d.dr1 * wdv;
+            for (int j = 0; j < 3; ++j) sm.D[el][q][10 + j][c] = ds3[j] * wdv;
+          }
+        }
+      __syncthreads();
+    }
+
+    // ---- D: residual rows (MResidual, PResidual, Stabilization)
+    if (valid)
       for (int k = l; k < ND; k += LPE) {
         const int n = k >> 2, i = k & 3;
         double r = 0.0;
@@ -275,45 +345,97 @@ __global__ __launch_bounds__(64) void assembly_kernel(AsmArgs a) {
             }
           }
         }
-        atomic_add(a.R + 4 * (int64_t)sm.conn[el][n] + i, r);
+        a.emr[e * ND + k] = r;
       }
 
-    // ---- Jacobian columns: C = D B for column (b,k), then K(row, col) = B^T C
+    // ---- E: Jacobian columns; lane owns column (b,kc)
     if constexpr (JAC) {
-      if (valid && a.vals)
+      if (valid)
         for (int col = l; col < ND; col += LPE) {
-          const int b = col >> 2, kc = col & 3;
-          double C[NQ][13];
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const double* shb = sm.sh[el][q][b];
-            for (int r = 0; r < 13; ++r) {
-              const double* Dr = sm.D[el][q][r];
-              if (kc < 3)
-                C[q][r] = Dr[3 * kc] * shb[1] + Dr[3 * kc + 1] * shb[2] + Dr[3 * kc + 2] * shb[3];
-              else
-                C[q][r] = Dr[9] * shb[0] + Dr[10] * shb[1] + Dr[11] * shb[2] + Dr[12] * shb[3];
-            }
-          }
-          for (int n = 0; n < NN; ++n) {
-            double v[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const double* sh = sm.sh[el][q][n];
-              for (int i = 0; i < 3; ++i) v[i] += C[q][3 * i] * sh[1] + C[q][3 * i + 1] * sh[2] + C[q][3 * i + 2] * sh[3];
-              v[3] += C[q][9] * sh[0] + C[q][10] * sh[1] + C[q][11] * sh[2] + C[q][12] * sh[3];
-            }
-            if (KIND == ASM_JACOBIAN) {
-              const int64_t off = 4 * (int64_t)sm.pos[el][n * NN + b] + kc;
-              for (int i = 0; i < 4; ++i) atomic_add(a.vals + sm.rowbase[el][4 * n + i] + off, v[i]);
-            } else {
-              // transposed scatter (Displacement<FADT>::scatter_adjoint,
-              // src/goal_displacement.cpp:196-214): entry (row, col) -> (col, row)
-              const int64_t base = sm.rowbase[el][col] + 4 * (int64_t)sm.pos[el][b * NN + n];
-              for (int i = 0; i < 4; ++i) atomic_add(a.vals + base + i, v[i]);
-            }
-          }
+          emit_column<NN, NQ>(sm.sh[el], sm.D[el], col, a.em + e * (ND * ND) + col);
         }
+    }
+    __syncthreads();
+  }
+}
+
+// One wavefront per mesh node a: rows 4a..4a+3 of the CSR matrix are
+// contiguous (16*deg(a) entries starting at row_ptr[4a]).  The wave reads the
+// row blocks (primal) or column blocks (adjoint) of up to 64 incident
+// elements at a time, all loads independent, adds them into an LDS copy of
+// the node's rows with LDS atomics (fixed instruction and lane order, hence a
+// fixed summation order), then stores the rows once.
+template <int NN>
+__global__ __launch_bounds__(64) void gather_kernel(GatherArgs g) {
+  constexpr int ND = 4 * NN, RB = 4 * ND;
+  extern __shared__ double acc[];
+  __shared__ double racc[4];
+  __shared__ int32_t sinc[64];
+  __shared__ uint16_t spos[64][NN];
+  const int lane = threadIdx.x;
+  for (int64_t node = blockIdx.x; node < g.num_nodes; node += gridDim.x) {
+    const int64_t r0 = g.row_ptr[4 * node];
+    const int deg4 = (int)(g.row_ptr[4 * node + 1] - r0);  // 4 * deg(node)
+    const int blk = 4 * deg4;
+    if (g.vals)
+      for (int t = lane; t < blk; t += 64) acc[t] = 0.0;
+    if (lane < 4) racc[lane] = 0.0;
+    __syncthreads();
+    const int64_t k0 = g.inc_ptr[node], k1 = g.inc_ptr[node + 1];
+    for (int64_t kb = k0; kb < k1; kb += 64) {
+      const int m = (int)min((int64_t)64, k1 - kb);
+      if (lane < m) sinc[lane] = g.inc[kb + lane];
+      __syncthreads();
+      if (g.vals) {
+        for (int t = lane; t < m * NN; t += 64) {
+          const int32_t pk = sinc[t / NN];
+          spos[t / NN][t % NN] = g.pos[(int64_t)pk * NN + t % NN];
+        }
+        __syncthreads();
+      }
+      for (int v = lane; v < 4 * m; v += 64) {
+        const int32_t pk = sinc[v >> 2];
+        atomicAdd(&racc[v & 3], g.emr[(int64_t)(pk / NN) * ND + 4 * (pk % NN) + (v & 3)]);
+      }
+      if (g.vals) {
+        const int total = m * RB;
+        for (int v0 = 0; v0 < total; v0 += 8 * 64) {
+          double src[8];
+          int slot[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int v = v0 + u * 64 + lane;
+            slot[u] = -1;
+            src[u] = 0.0;
+            if (v < total) {
+              const int j = v / RB, t = v % RB;
+              const int32_t pk = sinc[j];
+              const int64_t e = pk / NN;
+              const int la = pk % NN;
+              const int i = t / ND, c = t % ND;
+              const double* em = g.em + e * (ND * ND);
+              src[u] = g.adjoint ? em[c * ND + 4 * la + i] : em[(4 * la + i) * ND + c];
+              slot[u] = i * deg4 + 4 * (int)spos[j][c >> 2] + (c & 3);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (slot[u] >= 0) atomicAdd(&acc[slot[u]], src[u]);
+        }
+      }
+      __syncthreads();  // sinc is rewritten by the next chunk
+    }
+    __syncthreads();
+    if (g.R && lane < 4) {
+      const int64_t row = 4 * node + lane;
+      g.R[row] = (g.overwrite_R ? 0.0 : g.R[row]) + racc[lane];
+    }
+    if (g.vals) {
+      double* out = g.vals + r0;
+      if (g.overwrite_vals)
+        for (int t = lane; t < blk; t += 64) out[t] = acc[t];
+      else
+        for (int t = lane; t < blk; t += 64) out[t] += acc[t];
     }
     __syncthreads();
   }
@@ -323,10 +445,10 @@ template <int NN, int NQ, int MODEL>
 hipError_t launch3(int kind, const AsmArgs& a, int blocks, hipStream_t s) {
   dim3 grid(blocks), block(64);
   switch (kind) {
-    case ASM_RESIDUAL: hipLaunchKernelGGL((assembly_kernel<NN, NQ, MODEL, ASM_RESIDUAL>), grid, block, 0, s, a); break;
-    case ASM_JACOBIAN: hipLaunchKernelGGL((assembly_kernel<NN, NQ, MODEL, ASM_JACOBIAN>), grid, block, 0, s, a); break;
-    case ASM_ADJOINT: hipLaunchKernelGGL((assembly_kernel<NN, NQ, MODEL, ASM_ADJOINT>), grid, block, 0, s, a); break;
-    case ASM_LOCALIZE: hipLaunchKernelGGL((assembly_kernel<NN, NQ, MODEL, ASM_LOCALIZE>), grid, block, 0, s, a); break;
+    case ASM_RESIDUAL: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_RESIDUAL>), grid, block, 0, s, a); break;
+    case ASM_JACOBIAN: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_JACOBIAN>), grid, block, 0, s, a); break;
+    case ASM_ADJOINT: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_JACOBIAN>), grid, block, 0, s, a); break;
+    case ASM_LOCALIZE: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_LOCALIZE>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -379,7 +501,7 @@ __global__ __launch_bounds__(256) void qoi_kernel(QoiArgs a) {
         const int64_t base = 4 * (int64_t)ec[n];
         s += (a.x[base] + a.x[base + 1] + a.x[base + 2]) * N;
         if (a.dMdu)
-          for (int i = 0; i < 3; ++i) atomic_add(a.dMdu + base + i, N * wdv / 3.0);
+          for (int i = 0; i < 3; ++i) unsafeAtomicAdd(a.dMdu + base + i, N * wdv / 3.0);
       }
       val += s * wdv / 3.0;
       continue;
@@ -435,9 +557,9 @@ __global__ __launch_bounds__(256) void qoi_kernel(QoiArgs a) {
       for (int n = 0; n < NN; ++n) {
         const int64_t base = 4 * (int64_t)ec[n];
         for (int k = 0; k < 3; ++k)
-          atomic_add(a.dMdu + base + k,
-                     (gF[3 * k] * dNs[n][0] + gF[3 * k + 1] * dNs[n][1] + gF[3 * k + 2] * dNs[n][2]) * wdv);
-        atomic_add(a.dMdu + base + 3, gp * Ns[n] * wdv);
+          unsafeAtomicAdd(a.dMdu + base + k,
+                          (gF[3 * k] * dNs[n][0] + gF[3 * k + 1] * dNs[n][1] + gF[3 * k + 2] * dNs[n][2]) * wdv);
+        unsafeAtomicAdd(a.dMdu + base + 3, gp * Ns[n] * wdv);
       }
     }
   }
@@ -459,7 +581,7 @@ hipError_t launch_qoi2(const QoiArgs& a, hipStream_t s) {
 
 }  // namespace
 
-hipError_t launch_assembly(int kind, int order, int quad_order, const AsmArgs& a, int blocks, hipStream_t s) {
+hipError_t launch_element(int kind, int order, int quad_order, const AsmArgs& a, int blocks, hipStream_t s) {
   if (a.num_elems == 0) return hipSuccess;
   if (blocks < 1) return hipErrorInvalidValue;
   if (order == 1 && quad_order == 1) return launch2<4, 1>(kind, a, blocks, s);
@@ -467,6 +589,17 @@ hipError_t launch_assembly(int kind, int order, int quad_order, const AsmArgs& a
   if (order == 2 && quad_order == 1) return launch2<10, 1>(kind, a, blocks, s);
   if (order == 2 && quad_order == 2) return launch2<10, 4>(kind, a, blocks, s);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_gather(int order, const GatherArgs& g, int blocks, hipStream_t s) {
+  if (g.num_nodes == 0) return hipSuccess;
+  if (blocks < 1) return hipErrorInvalidValue;
+  const size_t lds = g.vals ? sizeof(double) * 16 * (size_t)g.max_degree : 0;
+  if (order == 1)
+    hipLaunchKernelGGL(gather_kernel<4>, dim3(blocks), dim3(64), lds, s, g);
+  else
+    hipLaunchKernelGGL(gather_kernel<10>, dim3(blocks), dim3(64), lds, s, g);
+  return hipGetLastError();
 }
 
 hipError_t launch_qoi(int order, int quad_order, const QoiArgs& a, hipStream_t s) {

@@ -160,9 +160,9 @@ struct goal_disc_s {
   int max_degree = 0;
   int64_t num_dofs = 0, nnz = 0;
   DevBuf<double> coords, x;
-  DevBuf<int32_t> conn, elem_set, cols, vertex_nodes;
+  DevBuf<int32_t> conn, elem_set, cols, vertex_nodes, inc;
   DevBuf<uint16_t> pos;
-  DevBuf<int64_t> row_ptr;
+  DevBuf<int64_t> row_ptr, inc_ptr;
   std::map<std::string, std::unique_ptr<DevBuf<int32_t>>> dev_node_sets;
   DevBuf<double> dbc_values;
 };
@@ -174,9 +174,12 @@ struct goal_states_s {
   DevBuf<double> sigma, eqps, eqps_old, Fp, Fp_old;
 };
 
+// zero_* only mark a buffer as pending zero; the next assembly then writes
+// it instead of adding to it, and any other access fills it first.
 struct goal_solinfo_s {
   goal_disc d = nullptr;
   DevBuf<double> R, dMdu, vals;
+  bool zero_R = false, zero_dMdu = false, zero_vals = false;
 };
 
 struct goal_mech_s {
@@ -187,6 +190,7 @@ struct goal_mech_s {
   DevBuf<goal::MatParams> mats;
   DevBuf<int32_t> status;
   DevBuf<double> elem_value;
+  DevBuf<double> emr, em;  // element residuals / matrices between the two passes
 };
 
 struct goal_nested_s {
@@ -211,6 +215,8 @@ void disc_build(goal_disc d) {
   d->elem_set.upload(m.elem_set.data(), (int64_t)m.elem_set.size());
   d->vertex_nodes.upload(m.vertex_nodes.data(), (int64_t)m.vertex_nodes.size());
   d->pos.upload(g.pos.data(), (int64_t)g.pos.size());
+  d->inc_ptr.upload(g.inc_ptr.data(), (int64_t)g.inc_ptr.size());
+  d->inc.upload(g.inc.data(), (int64_t)g.inc.size());
   DevBuf<int64_t> adj_ptr;
   DevBuf<int32_t> adj;
   adj_ptr.upload(g.node_adj_ptr.data(), (int64_t)g.node_adj_ptr.size());
@@ -264,8 +270,6 @@ goal::AsmArgs asm_args(goal_mech m) {
   a.x = d->x.p;
   a.elem_set = d->elem_set.p;
   a.mats = m->mats.p;
-  a.row_ptr = d->row_ptr.p;
-  a.pos = d->pos.p;
   a.model = m->model;
   a.stabilization = m->stab;
   a.xcd_remap = m->xcd_remap;
@@ -289,6 +293,56 @@ int asm_blocks(goal_mech m) {
   want = std::min<int64_t>(want, std::max<int64_t>(groups, 1));
   want = std::max<int64_t>(8, (want + 7) / 8 * 8);
   return (int)want;
+}
+
+int gather_blocks(goal_disc d) {
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx().device);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(d->mesh.num_nodes, (int64_t)cus * 32));
+}
+
+// Fill whatever zero_* left pending (before any access other than assembly).
+void materialize(goal_solinfo s) {
+  if (s->zero_R) HIPCHK(goal::launch_fill(s->R.p, s->R.n, 0.0, stream()));
+  if (s->zero_dMdu) HIPCHK(goal::launch_fill(s->dMdu.p, s->dMdu.n, 0.0, stream()));
+  if (s->zero_vals) HIPCHK(goal::launch_fill(s->vals.p, s->vals.n, 0.0, stream()));
+  s->zero_R = s->zero_dMdu = s->zero_vals = false;
+}
+
+// Element pass + node gather.  want_vals: Jacobian assembly.
+void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool want_vals) {
+  goal_disc d = m->d;
+  const int64_t nd = 4 * (int64_t)d->mesh.nodes_per_elem;
+  if (m->emr.n < d->mesh.num_elems * nd) m->emr.alloc(d->mesh.num_elems * nd);
+  if (want_vals && m->em.n < d->mesh.num_elems * nd * nd) m->em.alloc(d->mesh.num_elems * nd * nd);
+  a.emr = m->emr.p;
+  a.em = want_vals ? m->em.p : nullptr;
+  HIPCHK(goal::launch_element(kind, d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
+  if (s->zero_dMdu) {
+    HIPCHK(goal::launch_fill(s->dMdu.p, s->dMdu.n, 0.0, stream()));
+    s->zero_dMdu = false;
+  }
+  if (!want_vals && s->zero_vals) {
+    HIPCHK(goal::launch_fill(s->vals.p, s->vals.n, 0.0, stream()));
+    s->zero_vals = false;
+  }
+  goal::GatherArgs g{};
+  g.num_nodes = d->mesh.num_nodes;
+  g.inc_ptr = d->inc_ptr.p;
+  g.inc = d->inc.p;
+  g.pos = d->pos.p;
+  g.row_ptr = d->row_ptr.p;
+  g.emr = m->emr.p;
+  g.em = want_vals ? m->em.p : nullptr;
+  g.R = s->R.p;
+  g.vals = want_vals ? s->vals.p : nullptr;
+  g.max_degree = d->max_degree;
+  g.adjoint = kind == goal::ASM_ADJOINT ? 1 : 0;
+  g.overwrite_R = s->zero_R ? 1 : 0;
+  g.overwrite_vals = (want_vals && s->zero_vals) ? 1 : 0;
+  HIPCHK(goal::launch_gather(d->mesh.order, g, gather_blocks(d), stream()));
+  s->zero_R = false;
+  if (want_vals) s->zero_vals = false;
 }
 
 }  // namespace
@@ -510,6 +564,25 @@ int goal_disc_get_graph(goal_disc d, int64_t* row_ptr, int32_t* cols) {
   });
 }
 
+int goal_disc_graph_dev(goal_disc d, int64_t** row_ptr, int32_t** cols) {
+  return guard([&]() -> int {
+    require(d != nullptr, "null disc");
+    if (row_ptr) *row_ptr = d->row_ptr.p;
+    if (cols) *cols = d->cols.p;
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_mesh_dev(goal_disc d, double** coords, int32_t** conn, uint16_t** pos) {
+  return guard([&]() -> int {
+    require(d != nullptr, "null disc");
+    if (coords) *coords = d->coords.p;
+    if (conn) *conn = d->conn.p;
+    if (pos) *pos = d->pos.p;
+    return GOAL_OK;
+  });
+}
+
 int goal_disc_get_lids(goal_disc d, int32_t* lids) {
   return guard([&]() -> int {
     require(d && lids, "goal_disc_get_lids: bad arguments");
@@ -723,35 +796,36 @@ int goal_solinfo_destroy(goal_solinfo s) {
 int goal_solinfo_zero_R(goal_solinfo s) {
   return guard([&]() -> int {
     require(s != nullptr, "null solinfo");
-    HIPCHK(goal::launch_fill(s->R.p, s->R.n, 0.0, stream()));
+    s->zero_R = true;
     return GOAL_OK;
   });
 }
 int goal_solinfo_zero_dMdu(goal_solinfo s) {
   return guard([&]() -> int {
     require(s != nullptr, "null solinfo");
-    HIPCHK(goal::launch_fill(s->dMdu.p, s->dMdu.n, 0.0, stream()));
+    s->zero_dMdu = true;
     return GOAL_OK;
   });
 }
 int goal_solinfo_zero_dRdu(goal_solinfo s) {
   return guard([&]() -> int {
     require(s != nullptr, "null solinfo");
-    HIPCHK(goal::launch_fill(s->vals.p, s->vals.n, 0.0, stream()));
+    s->zero_vals = true;
     return GOAL_OK;
   });
 }
 int goal_solinfo_zero_all(goal_solinfo s) {
-  int st = goal_solinfo_zero_R(s);
-  if (st) return st;
-  st = goal_solinfo_zero_dRdu(s);
-  if (st) return st;
-  return goal_solinfo_zero_dMdu(s);
+  return guard([&]() -> int {
+    require(s != nullptr, "null solinfo");
+    s->zero_R = s->zero_vals = s->zero_dMdu = true;
+    return GOAL_OK;
+  });
 }
 
 int goal_solinfo_get(goal_solinfo s, double* R, double* dMdu, double* vals) {
   return guard([&]() -> int {
     require(s != nullptr, "null solinfo");
+    materialize(s);
     if (R) s->R.download(R, s->R.n);
     if (dMdu) s->dMdu.download(dMdu, s->dMdu.n);
     if (vals) s->vals.download(vals, s->vals.n);
@@ -762,6 +836,7 @@ int goal_solinfo_get(goal_solinfo s, double* R, double* dMdu, double* vals) {
 int goal_solinfo_set(goal_solinfo s, const double* R, const double* dMdu, const double* vals) {
   return guard([&]() -> int {
     require(s != nullptr, "null solinfo");
+    materialize(s);
     if (R) HIPCHK(hipMemcpyAsync(s->R.p, R, sizeof(double) * s->R.n, hipMemcpyHostToDevice, stream()));
     if (dMdu) HIPCHK(hipMemcpyAsync(s->dMdu.p, dMdu, sizeof(double) * s->dMdu.n, hipMemcpyHostToDevice, stream()));
     if (vals) HIPCHK(hipMemcpyAsync(s->vals.p, vals, sizeof(double) * s->vals.n, hipMemcpyHostToDevice, stream()));
@@ -773,6 +848,7 @@ int goal_solinfo_set(goal_solinfo s, const double* R, const double* dMdu, const 
 int goal_solinfo_dev(goal_solinfo s, double** R, double** dMdu, double** vals) {
   return guard([&]() -> int {
     require(s != nullptr, "null solinfo");
+    materialize(s);
     if (R) *R = s->R.p;
     if (dMdu) *dMdu = s->dMdu.p;
     if (vals) *vals = s->vals.p;
@@ -783,6 +859,7 @@ int goal_solinfo_dev(goal_solinfo s, double** R, double** dMdu, double** vals) {
 int goal_solinfo_norm_R(goal_solinfo s, double* norm) {
   return guard([&]() -> int {
     require(s && norm, "bad arguments");
+    materialize(s);
     HIPCHK(goal::launch_reduce(1, s->R.p, nullptr, s->R.n, scratch(), result_slot(0), stream()));
     double v = 0.0;
     HIPCHK(hipMemcpyAsync(&v, result_slot(0), sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -795,6 +872,7 @@ int goal_solinfo_norm_R(goal_solinfo s, double* norm) {
 int goal_solinfo_dot_R(goal_solinfo s, const double* z_dev, double* dot) {
   return guard([&]() -> int {
     require(s && z_dev && dot, "bad arguments");
+    materialize(s);
     HIPCHK(goal::launch_reduce(2, s->R.p, z_dev, s->R.n, scratch(), result_slot(0), stream()));
     double v = 0.0;
     HIPCHK(hipMemcpyAsync(&v, result_slot(0), sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -880,9 +958,8 @@ int goal_asm_residual(goal_mech m, goal_solinfo s, int save_states) {
   return guard([&]() -> int {
     require(m && s && s->d == m->d, "goal_asm_residual: bad arguments");
     goal::AsmArgs a = asm_args(m);
-    a.R = s->R.p;
     a.save_state = save_states ? 1 : 0;
-    HIPCHK(goal::launch_assembly(goal::ASM_RESIDUAL, m->d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
+    run_assembly(m, s, goal::ASM_RESIDUAL, a, false);
     return finish_assembly(m);
   });
 }
@@ -891,11 +968,8 @@ int goal_asm_jacobian(goal_mech m, goal_solinfo s, int adjoint, int save_states)
   return guard([&]() -> int {
     require(m && s && s->d == m->d, "goal_asm_jacobian: bad arguments");
     goal::AsmArgs a = asm_args(m);
-    a.R = s->R.p;
-    a.vals = s->vals.p;
     a.save_state = save_states ? 1 : 0;
-    HIPCHK(goal::launch_assembly(adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN, m->d->mesh.order, m->quad_order,
-                                 a, asm_blocks(m), stream()));
+    run_assembly(m, s, adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN, a, true);
     return finish_assembly(m);
   });
 }
@@ -904,11 +978,10 @@ int goal_asm_localize(goal_mech m, goal_solinfo s, const double* z_diff, const d
   return guard([&]() -> int {
     require(m && s && s->d == m->d && z_diff && z_coarse, "goal_asm_localize: bad arguments");
     goal::AsmArgs a = asm_args(m);
-    a.R = s->R.p;
     a.save_state = 0;
     a.z_diff = z_diff;
     a.z_coarse = z_coarse;
-    HIPCHK(goal::launch_assembly(goal::ASM_LOCALIZE, m->d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
+    run_assembly(m, s, goal::ASM_LOCALIZE, a, false);
     return finish_assembly(m);
   });
 }
@@ -932,6 +1005,7 @@ int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* va
     q.kind = kind;
     q.qoi_set = elem_set;
     q.elem_value = m->elem_value.p;
+    if (s) materialize(s);
     q.dMdu = s ? s->dMdu.p : nullptr;
     q.status = m->status.p;
     HIPCHK(goal::launch_qoi(d->mesh.order, m->quad_order, q, stream()));
@@ -955,6 +1029,7 @@ int goal_eval(const char* expr, double x, double y, double z, double t, double* 
 
 static int dbc_apply_dev(goal_disc d, goal_solinfo s, int comp, const int32_t* nodes_dev, int64_t n,
                          const std::vector<double>& values, int jacobian) {
+  materialize(s);
   d->dbc_values.upload(values.data(), n);
   HIPCHK(goal::launch_dbc(nodes_dev, d->dbc_values.p, n, comp, d->x.p, s->R.p, jacobian ? s->dMdu.p : nullptr,
                           d->row_ptr.p, d->cols.p, jacobian ? s->vals.p : nullptr, stream()));

@@ -319,7 +319,7 @@ struct J2Model {
     for (int k = 0; k < 9; ++k) s[k] = str[k];
     if (plastic) {
       double dX;
-      newton(0.0, 0.0, dgam, dX);
+      iters = newton(0.0, 0.0, dgam, dX);
       alpha = eqps_old + sq23 * dgam;
       for (int k = 0; k < 9; ++k) s[k] = str[k] - 2.0 * mubar * dgam * (str[k] / smag);
     }
@@ -331,7 +331,7 @@ struct J2Model {
   // The scalar Newton of src/goal_J2.cpp:98-121 together with the derivative
   // of each iterate along a direction in which |s| and mubar change by
   // dsmag and dmubar (eqps_old, Y, K are fixed).
-  __device__ inline void newton(double dsmag, double dmubar, double& X, double& dX) {
+  __device__ inline int newton(double dsmag, double dmubar, double& X, double& dX) const {
     const double sq23 = sqrt(2.0 / 3.0);
     X = 0.0;
     dX = 0.0;
@@ -354,7 +354,7 @@ struct J2Model {
       double res = fabs(R);
       if ((res < 1.0e-11) || (res / Y < 1.0e-11) || (res / f < 1.0e-11)) converged = true;
     }
-    iters = iter;
+    return iter;
   }
   __device__ inline bool failed() const { return plastic && iters >= 30; }
   __device__ inline void new_Fp(const double* Fp_old, double* Fpn) const {
@@ -368,7 +368,7 @@ struct J2Model {
   __device__ inline double r1(const QP& q) const { return q.p / kappa - 0.5 * (q.J - 1.0 / q.J); }
   // dF = e_k e_L^T with bebar = F Cp^{-1} F^T and G = F Cp^{-1}:
   //   d bebar_ij = d_ik G_jL + G_iL d_jk, with G_jL = (bebar F^{-T})_jL
-  __device__ inline void dir(const QP& q, int k, int L, Dir& d) {
+  __device__ inline void dir(const QP& q, int k, int L, Dir& d) const {
     double dJJ = q.Fi[3 * L + k];
     double dJ = q.J * dJJ;
     double dJm23 = -(2.0 / 3.0) * Jm23 * dJJ;
@@ -393,7 +393,7 @@ struct J2Model {
     double ds[9];
     if (plastic) {
       double X, dX;
-      newton(dsmag, dmubar, X, dX);
+      (void)newton(dsmag, dmubar, X, dX);
       for (int m = 0; m < 9; ++m) {
         double N = str[m] / smag;
         double dN = dstr[m] / smag - str[m] * dsmag / (smag * smag);
@@ -445,6 +445,61 @@ struct Elastic {
     d.dr1 = (k == L) ? -(2.0 * mu / 3.0 + lambda) : 0.0;
   }
 };
+
+// ------------------------------------------- neo-Hookean tangent, closed form
+// After Mixed the neo-Hookean stress is sigma = mu J^{-5/3} dev(F F^T) + p I,
+// so P = c A + p J F^{-T} with c = mu J^{-2/3} and A = F - tr(F F^T)/3 F^{-T}.
+// Differentiating along dF = e_k e_L^T (dJ = J Finv_Lk, dF^{-T}_iJ =
+// -Finv_Jk Finv_Li, d tr(F F^T) = 2 F_kL):
+//   dP_iJ = Finv_Lk (P_iJ - 5/3 c A_iJ) - Finv_Jk P_iL
+//           + c (d_ik d_JL + F_iL Finv_Jk - 2/3 F_kL Finv_Ji)
+//   dr1   = -1/2 (1 + 1/J^2) J Finv_Lk
+// and for the stabilization flux s3_j = tau J (Cinv g)_j, with h = Finv^T g
+// and v = Cinv g:
+//   ds3_j = tau (Finv_Lk J v_j - J (h_k Cinv_Lj + Finv_jk v_L)).
+// The reference differentiates the same formulas before simplification
+// (the bulk term 1/2 kappa (J - 1/J) is added by the model and removed again
+// by Mixed), so the two agree to rounding.
+struct NHBase {
+  double F[9], Fi[9], P[9], A[9], Cinv[9], v[3], h[3];
+  double J, c, tau;
+};
+
+__device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st, NHBase& b) {
+  double trb = 0.0;
+  for (int k = 0; k < 9; ++k) trb += q.F[k] * q.F[k];
+  for (int k = 0; k < 9; ++k) {
+    b.F[k] = q.F[k];
+    b.Fi[k] = q.Fi[k];
+    b.P[k] = m.P[k];
+    b.Cinv[k] = st.Cinv[k];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) b.A[3 * i + j] = q.F[3 * i + j] - (trb / 3.0) * q.Fi[3 * j + i];
+  for (int j = 0; j < 3; ++j) {
+    b.v[j] = st.Cinv[3 * j] * q.g[0] + st.Cinv[3 * j + 1] * q.g[1] + st.Cinv[3 * j + 2] * q.g[2];
+    b.h[j] = q.Fi[j] * q.g[0] + q.Fi[3 + j] * q.g[1] + q.Fi[6 + j] * q.g[2];
+  }
+  b.J = q.J;
+  b.c = m.mu * q.J * m.Jm53;
+  b.tau = st.tau;
+}
+
+// Column c = 3k+L of the 13x13 tangent (rows: P_iJ, r1, s3_j), scaled by wdv.
+__device__ inline void nh_column(const NHBase& b, int k, int L, double wdv, double* out, int stride) {
+  const double FLk = b.Fi[3 * L + k];
+  for (int i = 0; i < 3; ++i)
+    for (int J = 0; J < 3; ++J) {
+      const double FJk = b.Fi[3 * J + k];
+      double d = FLk * (b.P[3 * i + J] - (5.0 / 3.0) * b.c * b.A[3 * i + J]) - FJk * b.P[3 * i + L] +
+                 b.c * (((i == k && J == L) ? 1.0 : 0.0) + b.F[3 * i + L] * FJk - (2.0 / 3.0) * b.F[3 * k + L] * b.Fi[3 * J + i]);
+      out[(3 * i + J) * stride] = d * wdv;
+    }
+  out[9 * stride] = -0.5 * (1.0 + 1.0 / (b.J * b.J)) * b.J * FLk * wdv;
+  for (int j = 0; j < 3; ++j)
+    out[(10 + j) * stride] =
+        b.tau * (FLk * b.J * b.v[j] - b.J * (b.h[k] * b.Cinv[3 * L + j] + b.Fi[3 * j + k] * b.v[L])) * wdv;
+}
 
 // tau = 1/2 c0 h^2 / mu with h^2 the mean squared edge length
 // (src/goal_stabilization.cpp:59-73).

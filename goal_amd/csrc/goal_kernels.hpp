@@ -33,10 +33,8 @@ struct AsmArgs {
   const double* x;          // [num_dofs] u,p interleaved per node
   const int32_t* elem_set;  // [ne]
   const MatParams* mats;    // [num_sets]
-  const int64_t* row_ptr;   // [num_dofs + 1]
-  const uint16_t* pos;      // [ne][npe][npe]
-  double* R;                // [num_dofs] or null
-  double* vals;             // [nnz] or null
+  double* emr;              // [ne][4*npe] element residuals
+  double* em;               // [ne][4*npe][4*npe] element matrices (row-major) or null
   // per quadrature point history, structure of arrays: comp * (ne*nq) + e*nq + q
   const double* Fp_old;
   const double* eqps_old;
@@ -51,6 +49,23 @@ struct AsmArgs {
   int save_state;
   int xcd_remap;            // 1: give each XCD a contiguous slab of elements
   int* status;              // bit 0: J2 return map failed
+};
+
+// Node-owner gather of element residuals / matrices into R and the CSR values.
+struct GatherArgs {
+  int64_t num_nodes;
+  const int64_t* inc_ptr;   // [num_nodes + 1] node -> incident elements
+  const int32_t* inc;       // packed e * npe + local node
+  const uint16_t* pos;      // [ne][npe][npe] position of node b in adj(node a)
+  const int64_t* row_ptr;   // [num_dofs + 1]
+  const double* emr;
+  const double* em;         // null: residual only
+  double* R;                // null: skip
+  double* vals;             // null: skip
+  int max_degree;
+  int adjoint;              // 1: scatter the transpose (dRdu^T)
+  int overwrite_R;          // 1: R = sum (R was pending zero), 0: R += sum
+  int overwrite_vals;
 };
 
 // QoI selection (src/goal_mechanics.cpp:148-167)
@@ -74,8 +89,8 @@ struct QoiArgs {
 };
 
 // Launchers (goal_assembly.hip).  order: 1 or 2; quad_order: 1 or 2.
-hipError_t launch_assembly(int kind, int order, int quad_order, const AsmArgs& a, int blocks,
-                           hipStream_t s);
+hipError_t launch_element(int kind, int order, int quad_order, const AsmArgs& a, int blocks, hipStream_t s);
+hipError_t launch_gather(int order, const GatherArgs& g, int blocks, hipStream_t s);
 hipError_t launch_qoi(int order, int quad_order, const QoiArgs& a, hipStream_t s);
 
 // Utility kernels (goal_util.hip)

@@ -226,6 +226,14 @@ HostGraph build_graph(const HostMesh& m, int num_threads) {
     }
   });
   if (g.max_degree > 65535) throw std::runtime_error("graph: node degree exceeds 65535");
+  if (ne * npe > (int64_t)INT32_MAX) throw std::runtime_error("graph: too many element-node incidences");
+  g.inc_ptr = inc_ptr;
+  g.inc.resize(inc.size());
+  {
+    std::vector<int64_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+      for (int a = 0; a < npe; ++a) g.inc[fill[m.conn[e * npe + a]]++] = (int32_t)(e * npe + a);
+  }
   g.num_dofs = nn * kNumEqs;
   g.nnz = g.node_adj_ptr[nn] * kNumEqs * kNumEqs;
 

@@ -53,6 +53,10 @@ struct HostGraph {
   // Per element, per ordered local node pair (a, b): position of node
   // conn[e][b] inside adj(conn[e][a]).  [num_elems][npe][npe]
   std::vector<uint16_t> pos;
+  // Node -> incident elements, packed as e * npe + local node index, in
+  // ascending element order.
+  std::vector<int64_t> inc_ptr;  // [num_nodes + 1]
+  std::vector<int32_t> inc;
 };
 
 // Kuhn (Freudenthal) box mesh: nx*ny*nz hexahedra on [0,lx]x[0,ly]x[0,lz],

@@ -92,6 +92,12 @@ int goal_disc_get_mesh(goal_disc d, double* coords, int32_t* conn, int32_t* elem
 /* Ghost graph after fillComplete: rows in lid order, sorted columns
  * (Disc::compute_graphs, src/goal_disc.cpp:307-332). row_ptr [num_dofs+1]. */
 int goal_disc_get_graph(goal_disc d, int64_t* row_ptr, int32_t* cols);
+/* Device pointers of the CSR graph (row_ptr [num_dofs+1] int64, cols [nnz] int32),
+ * for device-resident consumers such as a GPU linear solver. */
+int goal_disc_graph_dev(goal_disc d, int64_t** row_ptr_dev, int32_t** cols_dev);
+/* Device pointers of the mesh: coords [num_nodes][3], conn [num_elems][npe] and
+ * the element node-pair position table pos [num_elems][npe][npe] (uint16). */
+int goal_disc_mesh_dev(goal_disc d, double** coords_dev, int32_t** conn_dev, uint16_t** pos_dev);
 /* Disc::get_lids for every element, [num_elems][4*nodes_per_elem] (src/goal_disc.cpp:214-222) */
 int goal_disc_get_lids(goal_disc d, int32_t* lids);
 /* Node sets (Disc::get_nodes, src/goal_disc.cpp:179-182, compute_node_sets :378-396) */

@@ -57,3 +57,58 @@ def random_history(n_pts, seed, amp=0.02):
 def oracle_mesh_from(disc):
     coords, conn, es = disc.mesh()
     return O.Mesh(disc.order, coords, conn, es)
+
+
+def fan_mesh(order=1, subdiv=1, jitter=0.0, seed=0):
+    """Tets fanned around one centre node: an icosphere's triangles joined to the
+    origin.  subdiv=1 gives 80 tets sharing node 0 (more than 64 incident
+    elements).  order 2 adds straight edge midpoints in apf edge order."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, dtype=float) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        mid = {}
+        F2 = []
+
+        def m(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = V[a] + V[b]
+                V.append(p / np.linalg.norm(p))
+                mid[key] = len(V) - 1
+            return mid[key]
+
+        for a, b, c in F:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            F2 += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = F2
+    rng = np.random.default_rng(seed)
+    coords = np.vstack([np.zeros(3)] + [v * (1.0 + jitter * rng.uniform(-1, 1)) for v in V])
+    conn = []
+    for a, b, c in F:
+        tet = [0, a + 1, b + 1, c + 1]
+        X = coords[tet]
+        if np.linalg.det(np.array([X[1] - X[0], X[2] - X[0], X[3] - X[0]])) < 0:
+            tet[2], tet[3] = tet[3], tet[2]
+        conn.append(tet)
+    conn = np.array(conn, dtype=np.int32)
+    if order == 1:
+        return coords, conn
+    edges = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+    emap = {}
+    pts = list(coords)
+    conn2 = []
+    for tet in conn:
+        row = list(tet)
+        for a, b in edges:
+            key = (min(tet[a], tet[b]), max(tet[a], tet[b]))
+            if key not in emap:
+                emap[key] = len(pts)
+                pts.append(0.5 * (coords[key[0]] + coords[key[1]]))
+            row.append(emap[key])
+        conn2.append(row)
+    return np.array(pts), np.array(conn2, dtype=np.int32)

@@ -199,3 +199,55 @@ def test_norm_and_dot():
     assert abs(si.norm_R() - np.linalg.norm(R)) < 1e-13 * np.linalg.norm(R)
     z = g.DeviceArray.from_host(rng.standard_normal(disc.num_dofs))
     assert abs(si.dot_R(z.ptr) + R @ z.numpy()) < 1e-12 * np.abs(R).sum()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_high_incidence_node(order):
+    """Node 0 belongs to 80 elements: the gather handles it in two chunks."""
+    from tests.helpers import fan_mesh
+    quad = 1 if order == 1 else 2
+    coords, conn = fan_mesh(order, subdiv=1, jitter=0.05, seed=3)
+    disc = g.Disc.from_arrays(order, coords, conn)
+    om = O.Mesh(order, coords, conn)
+    assert (conn == 0).sum() == 80
+    for model in (1, 2):
+        m = MATS[model]
+        st = g.States(disc, model, quad)
+        mech = g.Mechanics(disc, st, [g.MaterialSpec(**m)], quad_order=quad)
+        x = random_state(disc.num_nodes, 7)
+        disc.set_soln(x)
+        si = g.SolInfo(disc)
+        for adjoint in (False, True):
+            si.zero_all()
+            mech.jacobian(si, adjoint=adjoint, save_states=False)
+            ost = O.States(om, model, quad)
+            kind = O.K_ADJOINT if adjoint else O.K_JACOBIAN
+            Ro, Vo = O.assemble(kind, om, model, [mat_row(m)], x, ost, quad_order=quad, save=False)
+            assert rel(si.R(), Ro) < RTOL
+            assert rel(si.vals(), Vo) < RTOL
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_assembly_bitwise_deterministic(order):
+    quad = 1 if order == 1 else 2
+    disc, st, mech, x, om, ost, mats = make(order, quad, 1, n=4)
+    si = g.SolInfo(disc)
+    out = []
+    for _ in range(2):
+        si.zero_all()
+        mech.jacobian(si, save_states=False)
+        out.append((si.R(), si.vals()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_accumulate_without_zero():
+    """Assembly adds into R / dRdu when they were not zeroed (SolInfo semantics)."""
+    disc, st, mech, x, om, ost, mats = make(1, 1, 1)
+    si = g.SolInfo(disc)
+    si.zero_all()
+    mech.jacobian(si, save_states=False)
+    R1, V1 = si.R(), si.vals()
+    mech.jacobian(si, save_states=False)
+    assert rel(si.R(), 2 * R1) < 1e-15
+    assert rel(si.vals(), 2 * V1) < 1e-15
