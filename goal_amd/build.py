@@ -20,7 +20,7 @@ LIB = PKG / "libgoal_amd.so"
 ARCH = os.environ.get("GOAL_AMD_ARCH", "gfx950")
 
 # Minimum waves per SIMD requested for the element kernel (register cap).
-ELEM_WAVES = int(os.environ.get("GOAL_ELEM_WAVES", "2"))
+ELEM_WAVES = int(os.environ.get("GOAL_ELEM_WAVES", "3"))
 
 COMMON = [
     "-O3",
