@@ -138,6 +138,7 @@ _sig("goal_nested_destroy", _i32, _p)
 _sig("goal_nested_disc", _p, _p)
 _sig("goal_nested_get_maps", _i32, _p, _p, _p, _i64p)
 _sig("goal_nested_transfer_soln", _i32, _p)
+_sig("goal_nested_transfer_states", _i32, _p, _p, _p)
 _sig("goal_dwr_set_coarse", _i32, _p, _p)
 _sig("goal_dwr_subtract", _i32, _p, _p, _p, _i64)
 _sig("goal_dwr_compute_error", _i32, _p, _p, _p)
@@ -521,6 +522,9 @@ class Nested:
 
     def transfer_soln(self) -> None:
         check(lib.goal_nested_transfer_soln(self.h))
+
+    def transfer_states(self, base_states: "States", fine_states: "States") -> None:
+        check(lib.goal_nested_transfer_states(self.h, base_states.h, fine_states.h))
 
     def set_coarse(self, field: DeviceArray) -> None:
         check(lib.goal_dwr_set_coarse(self.h, field.ptr))

@@ -937,6 +937,8 @@ int goal_mech_set_option(goal_mech m, const char* key, int64_t value) {
       m->xcd_remap = value ? 1 : 0;
     } else if (k == "check") {
       m->check = value ? 1 : 0;
+    } else if (k == "stabilization") {
+      m->stab = value ? 1 : 0;
     } else {
       return set_error(GOAL_ERR_INVALID, "unknown option: %s", key);
     }
@@ -1115,6 +1117,23 @@ int goal_nested_transfer_soln(goal_nested n) {
     require(n != nullptr, "null nested");
     HIPCHK(goal::launch_copy(n->fine->x.p, n->base->x.p, 4 * n->num_old, stream()));
     HIPCHK(goal::launch_set_coarse(n->fine->x.p, n->edge_parents.p, n->num_old, n->num_new, stream()));
+    return GOAL_OK;
+  });
+}
+
+int goal_nested_transfer_states(goal_nested n, goal_states base, goal_states fine) {
+  return guard([&]() -> int {
+    require(n && base && fine, "bad arguments");
+    require(base->d == n->base && fine->d == n->fine, "states belong to other discs");
+    require(base->model == fine->model && base->nq == fine->nq, "states differ in model or quadrature");
+    const int64_t nf = n->fine->mesh.num_elems;
+    HIPCHK(goal::launch_transfer_state(base->sigma.p, fine->sigma.p, 9, nf, fine->nq, base->npts, stream()));
+    if (base->model == GOAL_MODEL_J2) {
+      HIPCHK(goal::launch_transfer_state(base->Fp.p, fine->Fp_old.p, 9, nf, fine->nq, base->npts, stream()));
+      HIPCHK(goal::launch_transfer_state(base->Fp.p, fine->Fp.p, 9, nf, fine->nq, base->npts, stream()));
+      HIPCHK(goal::launch_transfer_state(base->eqps.p, fine->eqps_old.p, 1, nf, fine->nq, base->npts, stream()));
+      HIPCHK(goal::launch_transfer_state(base->eqps.p, fine->eqps.p, 1, nf, fine->nq, base->npts, stream()));
+    }
     return GOAL_OK;
   });
 }

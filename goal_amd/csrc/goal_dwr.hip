@@ -98,7 +98,27 @@ __global__ __launch_bounds__(kBlock) void set_error_kernel(const double* child, 
   }
 }
 
+// child QP state := parent QP state (same quadrature point index)
+__global__ __launch_bounds__(kBlock) void transfer_state_kernel(const double* base, double* fine, int ncomp,
+                                                                int64_t num_fine, int nq, int64_t base_pts) {
+  const int64_t fine_pts = num_fine * nq;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < fine_pts * ncomp;
+       t += (int64_t)gridDim.x * kBlock) {
+    const int64_t k = t / fine_pts, pt = t % fine_pts;
+    const int64_t child = pt / nq, q = pt % nq;
+    fine[k * fine_pts + pt] = base[k * base_pts + (child >> 3) * nq + q];
+  }
+}
+
 }  // namespace
+
+hipError_t launch_transfer_state(const double* base, double* fine, int ncomp, int64_t num_fine, int nq,
+                                 int64_t base_pts, hipStream_t s) {
+  if (num_fine <= 0) return hipSuccess;
+  hipLaunchKernelGGL(transfer_state_kernel, dim3(grid_for(num_fine * nq * ncomp)), dim3(kBlock), 0, s, base, fine,
+                     ncomp, num_fine, nq, base_pts);
+  return hipGetLastError();
+}
 
 hipError_t launch_set_coarse(double* z, const int32_t* edge_parents, int64_t num_old, int64_t num_new,
                              hipStream_t s) {

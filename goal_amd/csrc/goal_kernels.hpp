@@ -118,5 +118,8 @@ hipError_t launch_compute_error(int order, const int32_t* conn, int64_t ne, cons
 hipError_t launch_sum_contribs(const int32_t* vertex_nodes, int64_t nv, const double* e, double* scratch,
                                double* out, hipStream_t s);
 hipError_t launch_set_error(const double* eta_child, int64_t num_base, double* eta_base, hipStream_t s);
+// fine[k][child*nq + q] = base[k][(child/8)*nq + q] for ncomp components
+hipError_t launch_transfer_state(const double* base, double* fine, int ncomp, int64_t num_fine, int nq,
+                                 int64_t base_pts, hipStream_t s);
 
 }  // namespace goal

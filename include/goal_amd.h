@@ -163,7 +163,8 @@ int goal_mech_create(goal_disc d, goal_states st, int num_sets, const goal_mater
                      int quad_order, goal_mech* out);
 int goal_mech_destroy(goal_mech m);
 /* Options: "blocks" (workgroups per launch, 0 = auto), "xcd_remap" (0/1),
- * "check" (1: synchronise after each assembly and report J2 failures). */
+ * "check" (1: synchronise after each assembly and report J2 failures),
+ * "stabilization" (0/1, overrides the value given at creation). */
 int goal_mech_set_option(goal_mech m, const char* key, int64_t value);
 /* Accumulated kernel status bits since the last call (1: J2 return map failed). */
 int goal_mech_status(goal_mech m, int* flags);
@@ -212,6 +213,11 @@ goal_disc goal_nested_disc(goal_nested n);
 int goal_nested_get_maps(goal_nested n, int32_t* parent, int32_t* edge_parents, int64_t* num_new_vertices);
 /* Copy base u,p to the old vertices of the nested solution and interpolate new ones. */
 int goal_nested_transfer_soln(goal_nested n);
+/* State transfer onto the nested mesh: each child element takes its parent's
+ * current history (sigma, Fp, eqps), which become the nested "_old" fields as
+ * in the reference (fields copied to the refined mesh, then States::update in
+ * make_states, src/goal_mechanics.cpp:87-95, src/goal_states.cpp:130-141). */
+int goal_nested_transfer_states(goal_nested n, goal_states base, goal_states fine);
 /* Nested::set_coarse on a nested nodal field, in place (src/goal_nested.cpp:369-393) */
 int goal_dwr_set_coarse(goal_nested n, double* field_dev);
 /* NestedAdjoint::subtract (src/goal_nested_adjoint.cpp:182-197): out = fine - coarse */

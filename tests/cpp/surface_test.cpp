@@ -1,0 +1,109 @@
+// Drives goal_amd through its C++ surface (include/goal_amd.hpp) the way the
+// reference's GoalPrimal / GoalNested drivers drive goal (src/main_primal.cpp,
+// src/main_nested.cpp): Newton load steps, the functional, one nested-adjoint
+// error estimate.  The linear solves use a dense LU, adequate for the small
+// meshes this program is run on.  Prints one JSON object.
+//
+// usage: surface_test <order> <quad_order> <model> <n> <steps>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "goal_amd.hpp"
+
+namespace {
+
+// Dense LU with partial pivoting of the device CSR system A x = b.
+int dense_solve(goal::Disc* d, goal::SolInfo* s, const double* b_dev, double* x_dev) {
+  const int64_t n = d->get_num_dofs();
+  std::vector<int64_t> rp(n + 1);
+  std::vector<int32_t> cols(d->get_nnz());
+  goal::check(goal_disc_get_graph(d->handle(), rp.data(), cols.data()));
+  std::vector<double> vals = s->dRdu_values();
+  std::vector<double> A((size_t)(n * n), 0.0), b(n);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) A[r * n + cols[k]] = vals[k];
+  goal::check(goal_memcpy_d2h(b.data(), b_dev, 8 * n));
+  std::vector<int64_t> piv(n);
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t p = k;
+    for (int64_t i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > std::fabs(A[p * n + k])) p = i;
+    if (A[p * n + k] == 0.0) return 1;
+    if (p != k) {
+      for (int64_t j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+      std::swap(b[k], b[p]);
+    }
+    for (int64_t i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      if (f == 0.0) continue;
+      for (int64_t j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  std::vector<double> x(n);
+  for (int64_t i = n - 1; i >= 0; --i) {
+    double acc = b[i];
+    for (int64_t j = i + 1; j < n; ++j) acc -= A[i * n + j] * x[j];
+    x[i] = acc / A[i * n + i];
+  }
+  goal::check(goal_memcpy_h2d(x_dev, x.data(), 8 * n));
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 6) {
+    std::fprintf(stderr, "usage: %s order quad model n steps\n", argv[0]);
+    return 2;
+  }
+  const int order = std::atoi(argv[1]), quad = std::atoi(argv[2]), model = std::atoi(argv[3]);
+  const int n = std::atoi(argv[4]), steps = std::atoi(argv[5]);
+  try {
+    auto disc = goal::Disc::box(n, n, n, 1.0, 1.0, 1.0, order, 0.1, 11);
+    goal::MechanicsParams mp;
+    mp.model = model == 2 ? "J2" : model == 0 ? "elastic" : "neohookean";
+    mp.quad_order = quad;
+    goal_material mat{};
+    if (model == 2) mat = goal_material{GOAL_MODEL_J2, 200.0, 0.3, 2.0, 0.25, 1.0};
+    else mat = goal_material{GOAL_MODEL_NEOHOOKEAN, 1000.0, 0.25, 0.0, 0.0, 1.0};
+    mp.materials = {mat};
+    goal::Mechanics mech(mp, disc.get());
+    goal::PrimalParams pp;
+    pp.dirichlet = {{0, "zmin", "0.0"}, {1, "zmin", "0.0"}, {2, "zmin", "0.0"}, {2, "zmax", "0.01*t"}};
+    pp.max_iters = 10;
+    pp.tolerance = 1e-10;
+    goal::Primal primal(pp, &mech, dense_solve);
+    primal.build_data();
+    goal::Functional functional("avg disp", -1, &primal);
+    std::printf("{\"iters\": [");
+    double t_old = 0.0;
+    double J = 0.0;
+    for (int step = 1; step <= steps; ++step) {
+      const double t = step;
+      const int it = primal.solve(t, t_old);
+      functional.compute(t, t_old);
+      J = functional.get_value();
+      std::printf("%s%d", step > 1 ? ", " : "", it);
+      if (step < steps) mech.update_states();
+      t_old = t;
+    }
+    std::printf("], \"J\": %.17g, \"norm\": %.6e", J, primal.last_norm());
+    if (order == 1) {
+      goal::NestedAdjoint adj(&mech, "avg disp", -1, pp.dirichlet, dense_solve);
+      std::vector<double> eta = adj.run(t_old, t_old - 1.0);
+      double s = 0.0;
+      for (double e : eta) s += e;
+      std::printf(", \"eta_sum\": %.17g, \"bound\": %.17g, \"zR\": %.17g", s, adj.bound(), adj.z_dot_R());
+    }
+    mech.update_states();
+    std::printf("}\n");
+  } catch (goal::Error const& e) {
+    std::fprintf(stderr, "goal error %d: %s\n", e.code, e.what());
+    return 1;
+  }
+  return 0;
+}
