@@ -15,16 +15,23 @@
 // writes the rows once.  No atomics, no zero fill, and the summation order is
 // fixed, so results are bitwise reproducible.
 //
-// Pass 1 work layout (wave64): a workgroup is one wavefront and processes EPW
-// elements at a time (4 for 4-node tets: 16 lanes each; 1 for 10-node tets).
-// Phases, separated by wave barriers, with all element data in LDS:
-//   A  gather connectivity, vertex coordinates, DOFs
-//   B  shape functions per (quadrature point, node)
-//   C1 quadrature-point state per quadrature point (kinematics, model, states)
-//   C2 one column of the 13x13 tangent per (quadrature point, direction)
-//   D  residual rows
-//   E  Jacobian: a lane owns one column (b,k), forms C = D B for one row
-//      group at a time and writes K(row, col) = B^T C for all rows.
+// Pass 1 work layout: a workgroup takes G elements at a time (6 ten-node or
+// 16 four-node tets) and one lane per column of their element matrices.  The
+// phases, separated by workgroup barriers, keep all element data in LDS and
+// hand out their tasks over all lanes:
+//   A  gather connectivity, vertex coordinates, DOFs        (element, DOF)
+//   B  shape functions                                       (element, point, node)
+//   C1 quadrature-point state: kinematics, model, states     (element, point)
+//   C2 one column of the 13x13 point tangent (J2, elastic)   (element, point, direction)
+//   D  residual rows                                         (element, row)
+//   E  Jacobian: a lane owns one column (b,k), forms C = D B
+//      and writes K(row, col) = B^T C for all rows           (element, column)
+// For neo-Hookean, C2 is skipped: phase E evaluates C directly from the
+// closed-form tangent (dev::nh_direction), which keeps the 13x13 tables out of
+// LDS and lets three workgroups share a CU.
+
+#include <algorithm>
+#include <type_traits>
 
 #include "goal_device.hpp"
 #include "goal_kernels.hpp"
@@ -49,7 +56,8 @@ struct ModelOf<MODEL_J2> {
   using type = dev::J2Model;
 };
 
-// What phase C1 leaves in LDS for the tangent columns of phase C2.
+// What phase C1 leaves in LDS for the tangent columns (phase C2, or phase E
+// for neo-Hookean).
 template <int MODEL>
 struct QPStore {
   dev::QP qp;
@@ -61,26 +69,56 @@ struct QPStore<MODEL_NEOHOOKEAN> {
   dev::NHBase b;
 };
 
+// Workgroup shape.  Phase E gives one lane to each column of an element
+// matrix, so a workgroup takes G elements and G*ND lanes, rounded up to whole
+// wavefronts: 6 ten-node tets on 256 lanes, 16 four-node tets on 256.  Four
+// waves per workgroup let three workgroups share a CU at W = 3 waves per SIMD
+// (the register budget); five-wave workgroups were only ever placed one per CU.
 template <int NN, int NQ, int MODEL>
-struct Smem {
-  using M = typename ModelOf<MODEL>::type;
-  static constexpr int EPW = (NN == 4) ? 4 : 1;
+struct Layout {
   static constexpr int ND = 4 * NN;
-  double x[EPW][ND];
-  double X[EPW][4][3];
-  double sh[EPW][NQ][NN][4];  // N_a, dN_a/dX
-  double wdv[EPW][NQ];
-  double sig[EPW][NQ][13];
-  double D[EPW][NQ][13][13];
-  QPStore<MODEL> qs[EPW][NQ];
-  double zq[EPW][NQ][4];      // z_diff (u,p) at qp            (localize)
-  double zgq[EPW][NQ][4][3];  // grad z_diff                   (localize)
-  double zcq[EPW][NQ][4];     // z_coarse p and its gradient   (localize)
-  double zd[EPW][ND];
-  double zc[EPW][NN];
-  int32_t conn[EPW][NN];
+  static constexpr int G = (NN == 10) ? 6 : (NQ == 1 ? 16 : 4);
+  static constexpr int T = ((G * ND + 63) / 64) * 64;
+  static constexpr int W = (MODEL == MODEL_J2) ? 1 : GOAL_ELEM_WAVES;
 };
 
+template <int NN, int NQ, int G>
+struct SBase {
+  double x[G][4 * NN];
+  double X[G][4][3];
+  double sh[G][NQ][NN][4];  // N_a, dN_a/dX
+  double wdv[G][NQ];
+  double sig[G][NQ][13];
+  int32_t conn[G][NN];
+};
+template <int NQ, int MODEL, int G>
+struct SJac {
+  double D[G][NQ][13][13];
+  QPStore<MODEL> qs[G][NQ];
+};
+// neo-Hookean: phase E contracts the closed-form tangent itself
+template <int NQ, int G>
+struct SJac<NQ, MODEL_NEOHOOKEAN, G> {
+  QPStore<MODEL_NEOHOOKEAN> qs[G][NQ];
+};
+template <int NN, int NQ, int G>
+struct SLoc {
+  double zq[G][NQ][4];      // z_diff (u,p) at qp
+  double zgq[G][NQ][4][3];  // grad z_diff
+  double zcq[G][NQ][4];     // z_coarse p and its gradient
+  double zd[G][4 * NN];
+  double zc[G][NN];
+};
+struct SNone {};
+
+template <int NN, int NQ, int MODEL, int KIND, int G>
+struct Smem {
+  using Ext = typename std::conditional<
+      KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT, SJac<NQ, MODEL, G>,
+      typename std::conditional<KIND == ASM_LOCALIZE, SLoc<NN, NQ, G>, SNone>::type>::type;
+  SBase<NN, NQ, G> b;
+  Ext ext;
+};
 // Element-group schedule.  Plain: group = block + it * gridDim.  XCD slabs:
 // workgroups are dispatched round-robin over the 8 XCDs, so block b runs on
 // XCD b % 8; XCD x is given the groups [x*per, (x+1)*per) and its blocks walk
@@ -136,224 +174,253 @@ __device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[
     for (int i = 0; i < 4; ++i) out[(4 * n + i) * ND] = acc[n][i];
 }
 
-// Waves per SIMD the compiler must leave room for: the J2 return map needs the
-// larger register budget.
-template <int MODEL>
-constexpr int elem_waves() {
-  return MODEL == MODEL_J2 ? 1 : GOAL_ELEM_WAVES;
+// neo-Hookean column (b, kc): C_q from the closed-form tangent contracted
+// with node b's shape function (dev::nh_direction), then the rows as above.
+template <int NN, int NQ>
+__device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* wdv,
+                                      const QPStore<MODEL_NEOHOOKEAN>* qs, int col, double* out) {
+  constexpr int ND = 4 * NN;
+  const int b = col >> 2, kc = col & 3;
+  double acc[NN][4];
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+    for (int i = 0; i < 4; ++i) acc[n][i] = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < NQ; ++q) {
+    const double w = wdv[q];
+    const double* shb = sh[q][b];
+    const double G[3] = {w * shb[1], w * shb[2], w * shb[3]};
+    double C[13];
+    dev::nh_direction(qs[q].b, kc, w * shb[0], G, C);
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      const double* sa = sh[q][n];
+      const double s0 = sa[0], s1 = sa[1], s2 = sa[2], s3 = sa[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[n][i] += C[3 * i] * s1 + C[3 * i + 1] * s2 + C[3 * i + 2] * s3;
+      acc[n][3] += C[9] * s0 + C[10] * s1 + C[11] * s2 + C[12] * s3;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+    for (int i = 0; i < 4; ++i) out[(4 * n + i) * ND] = acc[n][i];
 }
 
 template <int NN, int NQ, int MODEL, int KIND>
-__global__ __launch_bounds__(64, elem_waves<MODEL>()) void element_kernel(AsmArgs a) {
-  using S = Smem<NN, NQ, MODEL>;
-  using M = typename S::M;
-  constexpr int EPW = S::EPW, ND = S::ND, LPE = 64 / EPW;
+__global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>::W)) void element_kernel(AsmArgs a) {
+  using Lay = Layout<NN, NQ, MODEL>;
+  using M = typename ModelOf<MODEL>::type;
+  constexpr int G = Lay::G, ND = Lay::ND, T = Lay::T;
   constexpr bool JAC = (KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT);
   constexpr bool LOC = (KIND == ASM_LOCALIZE);
-  __shared__ S sm;
-  const int lane = threadIdx.x;
-  const int el = lane / LPE;
-  const int l = lane % LPE;
+  __shared__ Smem<NN, NQ, MODEL, KIND, G> sm;
+  auto& s = sm.b;
+  const int tid = threadIdx.x;
   const int64_t ne = a.num_elems;
-  const int64_t ngroups = (ne + EPW - 1) / EPW;
+  const int64_t ngroups = (ne + G - 1) / G;
   const int64_t nqtot = ne * NQ;
 
   for (int64_t it = 0;; ++it) {
     const int64_t grp = group_of(it, ngroups, a.xcd_remap);
     if (grp < 0) break;
-    const int64_t e = grp * EPW + el;
-    const bool valid = e < ne;
+    const int64_t e0 = grp * G;
+    const int nv = (int)min((int64_t)G, ne - e0);  // elements of this group
 
     // ---- A: gather (Displacement/Pressure::gather, src/goal_displacement.cpp:139-156)
-    if (valid) {
-      const int32_t* ec = a.conn + e * NN;
-      if (l < NN) sm.conn[el][l] = ec[l];
-      if (l < 12) sm.X[el][l / 3][l % 3] = a.coords[3 * (int64_t)ec[l / 3] + (l % 3)];
-      for (int k = l; k < ND; k += LPE) {
-        const int64_t dof = 4 * (int64_t)ec[k >> 2] + (k & 3);
-        sm.x[el][k] = a.x[dof];
-        if (LOC) sm.zd[el][k] = a.z_diff[dof];
-      }
-      if (LOC)
-        for (int k = l; k < NN; k += LPE) sm.zc[el][k] = a.z_coarse[4 * (int64_t)ec[k] + 3];
+    for (int t = tid; t < nv * NN; t += T) s.conn[t / NN][t % NN] = a.conn[e0 * NN + t];
+    __syncthreads();
+    for (int t = tid; t < nv * 12; t += T) {
+      const int el = t / 12, c = t % 12;
+      s.X[el][c / 3][c % 3] = a.coords[3 * (int64_t)s.conn[el][c / 3] + (c % 3)];
     }
+    for (int t = tid; t < nv * ND; t += T) {
+      const int el = t / ND, k = t % ND;
+      const int64_t dof = 4 * (int64_t)s.conn[el][k >> 2] + (k & 3);
+      s.x[el][k] = a.x[dof];
+      if constexpr (LOC) sm.ext.zd[el][k] = a.z_diff[dof];
+    }
+    if constexpr (LOC)
+      for (int t = tid; t < nv * NN; t += T) sm.ext.zc[t / NN][t % NN] = a.z_coarse[4 * (int64_t)s.conn[t / NN][t % NN] + 3];
     __syncthreads();
 
     // ---- B: shape functions at the quadrature points (u/uw/p/pw at_point)
-    if (valid)
-      for (int t = l; t < NQ * NN; t += LPE) {
-        const int q = t / NN, n = t % NN;
-        double gl[4][3];
-        const double det = dev::tet_gradlambda(sm.X[el], gl);
-        double xi[3], w;
-        Quad<NQ>::point(q, xi, w);
-        const double lam[4] = {1.0 - xi[0] - xi[1] - xi[2], xi[0], xi[1], xi[2]};
-        double N, dN[3];
-        dev::shape<NN>(n, lam, gl, N, dN);
-        sm.sh[el][q][n][0] = N;
-        sm.sh[el][q][n][1] = dN[0];
-        sm.sh[el][q][n][2] = dN[1];
-        sm.sh[el][q][n][3] = dN[2];
-        if (n == 0) sm.wdv[el][q] = w * det;
-      }
+    for (int t = tid; t < nv * NQ * NN; t += T) {
+      const int el = t / (NQ * NN), r = t % (NQ * NN);
+      const int q = r / NN, n = r % NN;
+      double gl[4][3];
+      const double det = dev::tet_gradlambda(s.X[el], gl);
+      double xi[3], w;
+      Quad<NQ>::point(q, xi, w);
+      const double lam[4] = {1.0 - xi[0] - xi[1] - xi[2], xi[0], xi[1], xi[2]};
+      double N, dN[3];
+      dev::shape<NN>(n, lam, gl, N, dN);
+      s.sh[el][q][n][0] = N;
+      s.sh[el][q][n][1] = dN[0];
+      s.sh[el][q][n][2] = dN[1];
+      s.sh[el][q][n][3] = dN[2];
+      if (n == 0) s.wdv[el][q] = w * det;
+    }
     __syncthreads();
 
-    // ---- C1: quadrature-point state (Kinematics, model, Mixed; states)
-    if (valid)
-      for (int q = l; q < NQ; q += LPE) {
-        const MatParams mp = a.mats[a.elem_set ? a.elem_set[e] : 0];
-        dev::QP qp;
-        for (int k = 0; k < 9; ++k) qp.H[k] = 0.0;
-        qp.p = 0.0;
-        qp.g[0] = qp.g[1] = qp.g[2] = 0.0;
+    // ---- C1: quadrature-point state (Kinematics, model, Mixed; states), one lane per point
+    for (int t = tid; t < nv * NQ; t += T) {
+      const int el = t / NQ, q = t % NQ;
+      const int64_t e = e0 + el;
+      const MatParams mp = a.mats[a.elem_set ? a.elem_set[e] : 0];
+      dev::QP qp;
+      for (int k = 0; k < 9; ++k) qp.H[k] = 0.0;
+      qp.p = 0.0;
+      qp.g[0] = qp.g[1] = qp.g[2] = 0.0;
 #pragma unroll 1
-        for (int n = 0; n < NN; ++n) {
-          const double N = sm.sh[el][q][n][0];
-          const double pn = sm.x[el][4 * n + 3];
-          qp.p += pn * N;
-          for (int j = 0; j < 3; ++j) {
-            const double dNj = sm.sh[el][q][n][1 + j];
-            qp.g[j] += pn * dNj;
-            for (int i = 0; i < 3; ++i) qp.H[3 * i + j] += sm.x[el][4 * n + i] * dNj;
+      for (int n = 0; n < NN; ++n) {
+        const double N = s.sh[el][q][n][0];
+        const double pn = s.x[el][4 * n + 3];
+        qp.p += pn * N;
+        for (int j = 0; j < 3; ++j) {
+          const double dNj = s.sh[el][q][n][1 + j];
+          qp.g[j] += pn * dNj;
+          for (int i = 0; i < 3; ++i) qp.H[3 * i + j] += s.x[el][4 * n + i] * dNj;
+        }
+      }
+      for (int k = 0; k < 9; ++k) qp.F[k] = qp.H[k] + ((k % 4 == 0) ? 1.0 : 0.0);
+      qp.J = dev::det3(qp.F);
+      dev::inv3(qp.F, qp.J, qp.Fi);
+      dev::Stab st;
+      st.init(qp, a.stabilization ? dev::stab_tau(s.X[el], mp) : 0.0);
+      const double wdv = s.wdv[el][q];
+      const int64_t sidx = e * NQ + q;
+      M m;
+      if constexpr (MODEL == MODEL_J2) {
+        double Fpo[9];
+        for (int k = 0; k < 9; ++k) Fpo[k] = a.Fp_old[k * nqtot + sidx];
+        m.init(qp, mp, Fpo, a.eqps_old[sidx]);
+        if (m.failed()) atomicOr(a.status, 1);
+        if (a.save_state) {
+          a.eqps[sidx] = m.alpha;
+          if (m.plastic) {
+            double Fpn[9];
+            m.new_Fp(Fpo, Fpn);
+            for (int k = 0; k < 9; ++k) a.Fp[k * nqtot + sidx] = Fpn[k];
           }
         }
-        for (int k = 0; k < 9; ++k) qp.F[k] = qp.H[k] + ((k % 4 == 0) ? 1.0 : 0.0);
-        qp.J = dev::det3(qp.F);
-        dev::inv3(qp.F, qp.J, qp.Fi);
-        dev::Stab st;
-        st.init(qp, a.stabilization ? dev::stab_tau(sm.X[el], mp) : 0.0);
-        const double wdv = sm.wdv[el][q];
-        const int64_t sidx = e * NQ + q;
-        M m;
-        if constexpr (MODEL == MODEL_J2) {
-          double Fpo[9];
-          for (int k = 0; k < 9; ++k) Fpo[k] = a.Fp_old[k * nqtot + sidx];
-          m.init(qp, mp, Fpo, a.eqps_old[sidx]);
-          if (m.failed()) atomicOr(a.status, 1);
-          if (a.save_state) {
-            a.eqps[sidx] = m.alpha;
-            if (m.plastic) {
-              double Fpn[9];
-              m.new_Fp(Fpo, Fpn);
-              for (int k = 0; k < 9; ++k) a.Fp[k * nqtot + sidx] = Fpn[k];
-            }
-          }
+      } else {
+        m.init(qp, mp);
+      }
+      if (a.save_state)
+        for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
+      for (int k = 0; k < 9; ++k) s.sig[el][q][k] = m.P[k] * wdv;
+      s.sig[el][q][9] = m.r1(qp) * wdv;
+      double s3[3];
+      st.value(s3);
+      for (int j = 0; j < 3; ++j) s.sig[el][q][10 + j] = s3[j] * wdv;
+      if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
+        dev::nh_base(qp, m, st, sm.ext.qs[el][q].b);
+      } else if constexpr (JAC) {
+        auto& D = sm.ext.D[el][q];
+        sm.ext.qs[el][q].qp = qp;
+        sm.ext.qs[el][q].st = st;
+        sm.ext.qs[el][q].m = m;
+        // p column: dsigma/dp = I (Mixed); grad-p columns: stabilization only
+        double dPp[9], dr1p;
+        if constexpr (MODEL == MODEL_ELASTIC) {
+          for (int k = 0; k < 9; ++k) dPp[k] = (k % 4 == 0) ? 1.0 : 0.0;
+          dr1p = 1.0;
         } else {
-          m.init(qp, mp);
+          for (int i = 0; i < 3; ++i)
+            for (int jj = 0; jj < 3; ++jj) dPp[3 * i + jj] = qp.J * qp.Fi[3 * jj + i];
+          dr1p = 1.0 / mp.kappa;
         }
-        if (a.save_state)
-          for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
-        for (int k = 0; k < 9; ++k) sm.sig[el][q][k] = m.P[k] * wdv;
-        sm.sig[el][q][9] = m.r1(qp) * wdv;
-        double s3[3];
-        st.value(s3);
-        for (int j = 0; j < 3; ++j) sm.sig[el][q][10 + j] = s3[j] * wdv;
-        if (JAC) {
-          if constexpr (MODEL == MODEL_NEOHOOKEAN) {
-            dev::nh_base(qp, m, st, sm.qs[el][q].b);
-          } else {
-            sm.qs[el][q].qp = qp;
-            sm.qs[el][q].st = st;
-            sm.qs[el][q].m = m;
-          }
-          // p column: dsigma/dp = I (Mixed); grad-p columns: stabilization only
-          double dPp[9], dr1p;
-          if constexpr (MODEL == MODEL_ELASTIC) {
-            for (int k = 0; k < 9; ++k) dPp[k] = (k % 4 == 0) ? 1.0 : 0.0;
-            dr1p = 1.0;
-          } else {
-            for (int i = 0; i < 3; ++i)
-              for (int jj = 0; jj < 3; ++jj) dPp[3 * i + jj] = qp.J * qp.Fi[3 * jj + i];
-            dr1p = 1.0 / mp.kappa;
-          }
-          for (int r = 0; r < 9; ++r) sm.D[el][q][r][9] = dPp[r] * wdv;
-          sm.D[el][q][9][9] = dr1p * wdv;
-          for (int r = 10; r < 13; ++r) sm.D[el][q][r][9] = 0.0;
-          for (int I = 0; I < 3; ++I) {
-            double ds3[3];
-            st.dir_g(qp, I, ds3);
-            for (int r = 0; r < 10; ++r) sm.D[el][q][r][10 + I] = 0.0;
-            for (int j = 0; j < 3; ++j) sm.D[el][q][10 + j][10 + I] = ds3[j] * wdv;
-          }
+        for (int r = 0; r < 9; ++r) D[r][9] = dPp[r] * wdv;
+        D[9][9] = dr1p * wdv;
+        for (int r = 10; r < 13; ++r) D[r][9] = 0.0;
+        for (int I = 0; I < 3; ++I) {
+          double ds3[3];
+          st.dir_g(qp, I, ds3);
+          for (int r = 0; r < 10; ++r) D[r][10 + I] = 0.0;
+          for (int j = 0; j < 3; ++j) D[10 + j][10 + I] = ds3[j] * wdv;
         }
-        if (LOC) {
-          for (int i = 0; i < 4; ++i) {
-            double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
-            for (int n = 0; n < NN; ++n) {
-              const double zn = sm.zd[el][4 * n + i];
-              z += zn * sm.sh[el][q][n][0];
-              for (int j = 0; j < 3; ++j) zg[j] += zn * sm.sh[el][q][n][1 + j];
-            }
-            sm.zq[el][q][i] = z;
-            for (int j = 0; j < 3; ++j) sm.zgq[el][q][i][j] = zg[j];
-          }
+      }
+      if constexpr (LOC) {
+        for (int i = 0; i < 4; ++i) {
           double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
           for (int n = 0; n < NN; ++n) {
-            const double zn = sm.zc[el][n];
-            z += zn * sm.sh[el][q][n][0];
-            for (int j = 0; j < 3; ++j) zg[j] += zn * sm.sh[el][q][n][1 + j];
+            const double zn = sm.ext.zd[el][4 * n + i];
+            z += zn * s.sh[el][q][n][0];
+            for (int j = 0; j < 3; ++j) zg[j] += zn * s.sh[el][q][n][1 + j];
           }
-          sm.zcq[el][q][0] = z;
-          for (int j = 0; j < 3; ++j) sm.zcq[el][q][1 + j] = zg[j];
+          sm.ext.zq[el][q][i] = z;
+          for (int j = 0; j < 3; ++j) sm.ext.zgq[el][q][i][j] = zg[j];
         }
+        double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
+        for (int n = 0; n < NN; ++n) {
+          const double zn = sm.ext.zc[el][n];
+          z += zn * s.sh[el][q][n][0];
+          for (int j = 0; j < 3; ++j) zg[j] += zn * s.sh[el][q][n][1 + j];
+        }
+        sm.ext.zcq[el][q][0] = z;
+        for (int j = 0; j < 3; ++j) sm.ext.zcq[el][q][1 + j] = zg[j];
       }
+    }
     __syncthreads();
 
-    // ---- C2: tangent columns d sig / d F_kL, one (qp, direction) per lane
-    if (JAC) {
-      if (valid)
-        for (int t = l; t < NQ * 9; t += LPE) {
-          const int q = t / 9, c = t % 9;
-          const double wdv = sm.wdv[el][q];
-          if constexpr (MODEL == MODEL_NEOHOOKEAN) {
-            dev::nh_column(sm.qs[el][q].b, c / 3, c % 3, wdv, &sm.D[el][q][0][c], 13);
-          } else {
-            dev::Dir d;
-            sm.qs[el][q].m.dir(sm.qs[el][q].qp, c / 3, c % 3, d);
-            double ds3[3];
-            sm.qs[el][q].st.dir(sm.qs[el][q].qp, c / 3, c % 3, ds3);
-            for (int r = 0; r < 9; ++r) sm.D[el][q][r][c] = d.dP[r] * wdv;
-            sm.D[el][q][9][c] = d.dr1 * wdv;
-            for (int j = 0; j < 3; ++j) sm.D[el][q][10 + j][c] = ds3[j] * wdv;
-          }
-        }
+    // ---- C2: tangent columns d sig / d F_kL, one (element, point, direction) per lane
+    if constexpr (JAC && MODEL != MODEL_NEOHOOKEAN) {
+      for (int t = tid; t < nv * NQ * 9; t += T) {
+        const int el = t / (NQ * 9), r = t % (NQ * 9);
+        const int q = r / 9, c = r % 9;
+        const double wdv = s.wdv[el][q];
+        auto& qs = sm.ext.qs[el][q];
+        dev::Dir d;
+        qs.m.dir(qs.qp, c / 3, c % 3, d);
+        double ds3[3];
+        qs.st.dir(qs.qp, c / 3, c % 3, ds3);
+        auto& D = sm.ext.D[el][q];
+        for (int rr = 0; rr < 9; ++rr) D[rr][c] = d.dP[rr] * wdv;
+        D[9][c] = d.dr1 * wdv;
+        for (int j = 0; j < 3; ++j) D[10 + j][c] = ds3[j] * wdv;
+      }
       __syncthreads();
     }
 
-    // ---- D: residual rows (MResidual, PResidual, Stabilization)
-    if (valid)
-      for (int k = l; k < ND; k += LPE) {
-        const int n = k >> 2, i = k & 3;
-        double r = 0.0;
-        for (int q = 0; q < NQ; ++q) {
-          const double* sg = sm.sig[el][q];
-          const double* sh = sm.sh[el][q][n];
-          if (!LOC) {
-            if (i < 3) r += sg[3 * i] * sh[1] + sg[3 * i + 1] * sh[2] + sg[3 * i + 2] * sh[3];
-            else r += sg[9] * sh[0] + sg[10] * sh[1] + sg[11] * sh[2] + sg[12] * sh[3];
+    // ---- D: residual rows (MResidual, PResidual, Stabilization), one (element, row) per lane
+    for (int t = tid; t < nv * ND; t += T) {
+      const int el = t / ND, k = t % ND;
+      const int n = k >> 2, i = k & 3;
+      double r = 0.0;
+      for (int q = 0; q < NQ; ++q) {
+        const double* sg = s.sig[el][q];
+        const double* sh = s.sh[el][q][n];
+        if constexpr (!LOC) {
+          if (i < 3) r += sg[3 * i] * sh[1] + sg[3 * i + 1] * sh[2] + sg[3 * i + 2] * sh[3];
+          else r += sg[9] * sh[0] + sg[10] * sh[1] + sg[11] * sh[2] + sg[12] * sh[3];
+        } else {
+          // DisplacementAdjoint / PressureAdjoint weights
+          // (src/goal_displacement_adjoint.cpp:37-53, src/goal_pressure_adjoint.cpp:38-49)
+          if (i < 3) {
+            const double z = sm.ext.zq[el][q][i];
+            const double* zg = sm.ext.zgq[el][q][i];
+            for (int J = 0; J < 3; ++J) r += sg[3 * i + J] * (zg[J] * sh[0] + z * sh[1 + J]);
           } else {
-            // DisplacementAdjoint / PressureAdjoint weights
-            // (src/goal_displacement_adjoint.cpp:37-53, src/goal_pressure_adjoint.cpp:38-49)
-            if (i < 3) {
-              const double z = sm.zq[el][q][i];
-              const double* zg = sm.zgq[el][q][i];
-              for (int J = 0; J < 3; ++J) r += sg[3 * i + J] * (zg[J] * sh[0] + z * sh[1 + J]);
-            } else {
-              r += sg[9] * sm.zq[el][q][3] * sh[0];
-              const double zc = sm.zcq[el][q][0];
-              for (int J = 0; J < 3; ++J) r += sg[10 + J] * (sm.zcq[el][q][1 + J] * sh[0] + zc * sh[1 + J]);
-            }
+            r += sg[9] * sm.ext.zq[el][q][3] * sh[0];
+            const double zc = sm.ext.zcq[el][q][0];
+            for (int J = 0; J < 3; ++J) r += sg[10 + J] * (sm.ext.zcq[el][q][1 + J] * sh[0] + zc * sh[1 + J]);
           }
         }
-        a.emr[e * ND + k] = r;
       }
+      a.emr[e0 * ND + t] = r;
+    }
 
-    // ---- E: Jacobian columns; lane owns column (b,kc)
-    if constexpr (JAC) {
-      if (valid)
-        for (int col = l; col < ND; col += LPE) {
-          emit_column<NN, NQ>(sm.sh[el], sm.D[el], col, a.em + e * (ND * ND) + col);
-        }
+    // ---- E: Jacobian, one (element, column) per lane
+    if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
+      for (int t = tid; t < nv * ND; t += T) {
+        const int el = t / ND, col = t % ND;
+        emit_column_nh<NN, NQ>(s.sh[el], s.wdv[el], sm.ext.qs[el], col, a.em + (e0 + el) * (ND * ND) + col);
+      }
+    } else if constexpr (JAC) {
+      for (int t = tid; t < nv * ND; t += T) {
+        const int el = t / ND, col = t % ND;
+        emit_column<NN, NQ>(s.sh[el], sm.ext.D[el], col, a.em + (e0 + el) * (ND * ND) + col);
+      }
     }
     __syncthreads();
   }
@@ -441,17 +508,48 @@ __global__ __launch_bounds__(64) void gather_kernel(GatherArgs g) {
   }
 }
 
+template <int NN, int NQ, int MODEL, int KIND>
+int occupancy() {
+  static const int occ = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(&element_kernel<NN, NQ, MODEL, KIND>), Layout<NN, NQ, MODEL>::T, 0) !=
+            hipSuccess ||
+        n < 1)
+      n = 1;
+    return n;
+  }();
+  return occ;
+}
+
+// Persistent grid: the workgroups that fit on the device at once (a multiple
+// of 8 so the XCD slab schedule applies), at most one per element group.
+template <int NN, int NQ, int MODEL, int KIND>
+int resident_grid(int64_t ne) {
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int64_t groups = (ne + Layout<NN, NQ, MODEL>::G - 1) / Layout<NN, NQ, MODEL>::G;
+  int64_t want = std::min<int64_t>((int64_t)cus * occupancy<NN, NQ, MODEL, KIND>(), groups);
+  return (int)std::max<int64_t>(8, (want + 7) / 8 * 8);
+}
+
+template <int NN, int NQ, int MODEL, int KIND>
+hipError_t launch4(const AsmArgs& a, int blocks, hipStream_t s) {
+  if (blocks <= 0) blocks = resident_grid<NN, NQ, MODEL, KIND>(a.num_elems);
+  hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, KIND>), dim3(blocks), dim3(Layout<NN, NQ, MODEL>::T), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int NN, int NQ, int MODEL>
 hipError_t launch3(int kind, const AsmArgs& a, int blocks, hipStream_t s) {
-  dim3 grid(blocks), block(64);
   switch (kind) {
-    case ASM_RESIDUAL: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_RESIDUAL>), grid, block, 0, s, a); break;
-    case ASM_JACOBIAN: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_JACOBIAN>), grid, block, 0, s, a); break;
-    case ASM_ADJOINT: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_JACOBIAN>), grid, block, 0, s, a); break;
-    case ASM_LOCALIZE: hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, ASM_LOCALIZE>), grid, block, 0, s, a); break;
+    case ASM_RESIDUAL: return launch4<NN, NQ, MODEL, ASM_RESIDUAL>(a, blocks, s);
+    case ASM_JACOBIAN: return launch4<NN, NQ, MODEL, ASM_JACOBIAN>(a, blocks, s);
+    case ASM_ADJOINT: return launch4<NN, NQ, MODEL, ASM_JACOBIAN>(a, blocks, s);
+    case ASM_LOCALIZE: return launch4<NN, NQ, MODEL, ASM_LOCALIZE>(a, blocks, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <int NN, int NQ>
@@ -583,7 +681,6 @@ hipError_t launch_qoi2(const QoiArgs& a, hipStream_t s) {
 
 hipError_t launch_element(int kind, int order, int quad_order, const AsmArgs& a, int blocks, hipStream_t s) {
   if (a.num_elems == 0) return hipSuccess;
-  if (blocks < 1) return hipErrorInvalidValue;
   if (order == 1 && quad_order == 1) return launch2<4, 1>(kind, a, blocks, s);
   if (order == 1 && quad_order == 2) return launch2<4, 4>(kind, a, blocks, s);
   if (order == 2 && quad_order == 1) return launch2<10, 1>(kind, a, blocks, s);

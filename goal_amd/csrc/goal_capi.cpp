@@ -283,17 +283,8 @@ goal::AsmArgs asm_args(goal_mech m) {
   return a;
 }
 
-int asm_blocks(goal_mech m) {
-  if (m->blocks > 0) return m->blocks;
-  int cus = 256;
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx().device);
-  const int epw = m->d->mesh.order == 1 ? 4 : 1;
-  int64_t groups = (m->d->mesh.num_elems + epw - 1) / epw;
-  int64_t want = (int64_t)cus * 24;
-  want = std::min<int64_t>(want, std::max<int64_t>(groups, 1));
-  want = std::max<int64_t>(8, (want + 7) / 8 * 8);
-  return (int)want;
-}
+// 0 lets the launcher size a persistent grid from the kernel's occupancy.
+int asm_blocks(goal_mech m) { return m->blocks; }
 
 int gather_blocks(goal_disc d) {
   int cus = 256;

@@ -461,44 +461,73 @@ struct Elastic {
 // (the bulk term 1/2 kappa (J - 1/J) is added by the model and removed again
 // by Mixed), so the two agree to rounding.
 struct NHBase {
-  double F[9], Fi[9], P[9], A[9], Cinv[9], v[3], h[3];
-  double J, c, tau;
+  double F[9], Fi[9], P[9], Q[9], Cinv[9], v[3], h[3];
+  double J, c, tau, beta, ikappa;  // beta = -1/2 (1 + 1/J^2) J
 };
 
 __device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st, NHBase& b) {
   double trb = 0.0;
   for (int k = 0; k < 9; ++k) trb += q.F[k] * q.F[k];
+  const double c = m.mu * q.J * m.Jm53;
   for (int k = 0; k < 9; ++k) {
     b.F[k] = q.F[k];
     b.Fi[k] = q.Fi[k];
     b.P[k] = m.P[k];
     b.Cinv[k] = st.Cinv[k];
   }
+  // Q = P - 5/3 c A,  A = F - (tr b / 3) F^{-T}
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) b.A[3 * i + j] = q.F[3 * i + j] - (trb / 3.0) * q.Fi[3 * j + i];
+    for (int j = 0; j < 3; ++j)
+      b.Q[3 * i + j] = m.P[3 * i + j] - (5.0 / 3.0) * c * (q.F[3 * i + j] - (trb / 3.0) * q.Fi[3 * j + i]);
   for (int j = 0; j < 3; ++j) {
     b.v[j] = st.Cinv[3 * j] * q.g[0] + st.Cinv[3 * j + 1] * q.g[1] + st.Cinv[3 * j + 2] * q.g[2];
     b.h[j] = q.Fi[j] * q.g[0] + q.Fi[3 + j] * q.g[1] + q.Fi[6 + j] * q.g[2];
   }
   b.J = q.J;
-  b.c = m.mu * q.J * m.Jm53;
+  b.c = c;
   b.tau = st.tau;
+  b.beta = -0.5 * (1.0 + 1.0 / (q.J * q.J)) * q.J;
+  b.ikappa = 1.0 / m.kappa;
 }
 
-// Column c = 3k+L of the 13x13 tangent (rows: P_iJ, r1, s3_j), scaled by wdv.
-__device__ inline void nh_column(const NHBase& b, int k, int L, double wdv, double* out, int stride) {
-  const double FLk = b.Fi[3 * L + k];
-  for (int i = 0; i < 3; ++i)
-    for (int J = 0; J < 3; ++J) {
-      const double FJk = b.Fi[3 * J + k];
-      double d = FLk * (b.P[3 * i + J] - (5.0 / 3.0) * b.c * b.A[3 * i + J]) - FJk * b.P[3 * i + L] +
-                 b.c * (((i == k && J == L) ? 1.0 : 0.0) + b.F[3 * i + L] * FJk - (2.0 / 3.0) * b.F[3 * k + L] * b.Fi[3 * J + i]);
-      out[(3 * i + J) * stride] = d * wdv;
+// The 13 tangent rows (P_iJ, r1, s3_j) in one direction of the element DOFs,
+// contracted with the shape function of node b.
+//   Displacement DOF k of node b (dF = e_k (x) G, G = dN_b/dX):
+//     dP_iJ = g_k Q_iJ + Finv_Jk (c (F G)_i - (P G)_i) + c d_ik G_J - 2/3 c (F G)_k Finv_Ji
+//     dr1   = beta g_k,   with g_k = sum_L G_L Finv_Lk
+//     ds3_j = tau J (g_k v_j - h_k (Cinv G)_j - Finv_jk (v . G))
+//   Pressure DOF of node b (dp = N_b, d grad p = G), Mixed and PResidual:
+//     dP_iJ = N_b J Finv_Ji,   dr1 = N_b / kappa,   ds3_j = tau J (Cinv G)_j
+// These are the columns of the point tangent summed against G; scaling G and
+// N_b by w dv scales the result.
+__device__ inline void nh_direction(const NHBase& b, int k, double N, const double* G, double* C) {
+  if (k < 3) {
+    double FG[3], PG[3], CG[3];
+    for (int i = 0; i < 3; ++i) {
+      FG[i] = b.F[3 * i] * G[0] + b.F[3 * i + 1] * G[1] + b.F[3 * i + 2] * G[2];
+      PG[i] = b.P[3 * i] * G[0] + b.P[3 * i + 1] * G[1] + b.P[3 * i + 2] * G[2];
+      CG[i] = b.Cinv[3 * i] * G[0] + b.Cinv[3 * i + 1] * G[1] + b.Cinv[3 * i + 2] * G[2];
     }
-  out[9 * stride] = -0.5 * (1.0 + 1.0 / (b.J * b.J)) * b.J * FLk * wdv;
-  for (int j = 0; j < 3; ++j)
-    out[(10 + j) * stride] =
-        b.tau * (FLk * b.J * b.v[j] - b.J * (b.h[k] * b.Cinv[3 * L + j] + b.Fi[3 * j + k] * b.v[L])) * wdv;
+    const double gk = b.Fi[k] * G[0] + b.Fi[3 + k] * G[1] + b.Fi[6 + k] * G[2];
+    const double u = (2.0 / 3.0) * b.c * FG[k];
+    for (int i = 0; i < 3; ++i) {
+      const double t = b.c * FG[i] - PG[i];
+      for (int J = 0; J < 3; ++J)
+        C[3 * i + J] = gk * b.Q[3 * i + J] + b.Fi[3 * J + k] * t - u * b.Fi[3 * J + i] + (i == k ? b.c * G[J] : 0.0);
+    }
+    C[9] = b.beta * gk;
+    const double vG = b.v[0] * G[0] + b.v[1] * G[1] + b.v[2] * G[2];
+    const double tJ = b.tau * b.J;
+    for (int j = 0; j < 3; ++j) C[10 + j] = tJ * (gk * b.v[j] - b.h[k] * CG[j] - b.Fi[3 * j + k] * vG);
+  } else {
+    const double NJ = N * b.J;
+    for (int i = 0; i < 3; ++i)
+      for (int J = 0; J < 3; ++J) C[3 * i + J] = NJ * b.Fi[3 * J + i];
+    C[9] = N * b.ikappa;
+    const double tJ = b.tau * b.J;
+    for (int j = 0; j < 3; ++j)
+      C[10 + j] = tJ * (b.Cinv[3 * j] * G[0] + b.Cinv[3 * j + 1] * G[1] + b.Cinv[3 * j + 2] * G[2]);
+  }
 }
 
 // tau = 1/2 c0 h^2 / mu with h^2 the mean squared edge length
