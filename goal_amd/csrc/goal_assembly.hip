@@ -90,6 +90,7 @@ struct SBase {
   double wdv[G][NQ];
   double sig[G][NQ][13];
   int32_t conn[G][NN];
+  int32_t rec[G][NN];
 };
 template <int NQ, int MODEL, int G>
 struct SJac {
@@ -141,9 +142,29 @@ __device__ inline int64_t group_of(int64_t it, int64_t ngroups, int remap) {
 //   C_q = D_q B_q[:, col]  (13 values),   K[(a,i)][col] = sum_q B_q[:, (a,i)]^T C_q.
 // Quadrature points are the outer loop so each shape-function row of the LDS
 // table is read once per column; the NN*4 sums stay in registers.
-template <int NN, int NQ>
-__device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[13][13], int col, double* out) {
+// Store column col = (b, kc) of an element matrix: into the row blocks of the
+// element's nodes (rows (a, i) at record rec[a]), or, transposed, as row kc of
+// node b's block.
+template <int NN, bool TR>
+__device__ inline void store_column(const double (*acc)[4], int col, const int32_t* rec, double* em) {
   constexpr int ND = 4 * NN;
+  if constexpr (!TR) {
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      double* o = em + (int64_t)rec[n] * (4 * ND) + col;
+      for (int i = 0; i < 4; ++i) o[i * ND] = acc[n][i];
+    }
+  } else {
+    double* o = em + (int64_t)rec[col >> 2] * (4 * ND) + (col & 3) * ND;
+#pragma unroll
+    for (int n = 0; n < NN; ++n)
+      for (int i = 0; i < 4; ++i) o[4 * n + i] = acc[n][i];
+  }
+}
+
+template <int NN, int NQ, bool TR>
+__device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[13][13], int col,
+                                   const int32_t* rec, double* em) {
   const int b = col >> 2, kc = col & 3;
   double acc[NN][4];
 #pragma unroll
@@ -169,17 +190,15 @@ __device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[
       acc[n][3] += C[9] * s0 + C[10] * s1 + C[11] * s2 + C[12] * s3;
     }
   }
-#pragma unroll
-  for (int n = 0; n < NN; ++n)
-    for (int i = 0; i < 4; ++i) out[(4 * n + i) * ND] = acc[n][i];
+  store_column<NN, TR>(acc, col, rec, em);
 }
 
 // neo-Hookean column (b, kc): C_q from the closed-form tangent contracted
 // with node b's shape function (dev::nh_direction), then the rows as above.
-template <int NN, int NQ>
+template <int NN, int NQ, bool TR>
 __device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* wdv,
-                                      const QPStore<MODEL_NEOHOOKEAN>* qs, int col, double* out) {
-  constexpr int ND = 4 * NN;
+                                      const QPStore<MODEL_NEOHOOKEAN>* qs, int col, const int32_t* rec,
+                                      double* em) {
   const int b = col >> 2, kc = col & 3;
   double acc[NN][4];
 #pragma unroll
@@ -201,9 +220,7 @@ __device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* w
       acc[n][3] += C[9] * s0 + C[10] * s1 + C[11] * s2 + C[12] * s3;
     }
   }
-#pragma unroll
-  for (int n = 0; n < NN; ++n)
-    for (int i = 0; i < 4; ++i) out[(4 * n + i) * ND] = acc[n][i];
+  store_column<NN, TR>(acc, col, rec, em);
 }
 
 template <int NN, int NQ, int MODEL, int KIND>
@@ -227,7 +244,10 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     const int nv = (int)min((int64_t)G, ne - e0);  // elements of this group
 
     // ---- A: gather (Displacement/Pressure::gather, src/goal_displacement.cpp:139-156)
-    for (int t = tid; t < nv * NN; t += T) s.conn[t / NN][t % NN] = a.conn[e0 * NN + t];
+    for (int t = tid; t < nv * NN; t += T) {
+      s.conn[t / NN][t % NN] = a.conn[e0 * NN + t];
+      s.rec[t / NN][t % NN] = a.rec[e0 * NN + t];
+    }
     __syncthreads();
     for (int t = tid; t < nv * 12; t += T) {
       const int el = t / 12, c = t % 12;
@@ -407,19 +427,20 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
           }
         }
       }
-      a.emr[e0 * ND + t] = r;
+      a.emr[(int64_t)s.rec[el][n] * 4 + i] = r;
     }
 
     // ---- E: Jacobian, one (element, column) per lane
+    constexpr bool TR = (KIND == ASM_ADJOINT);
     if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
       for (int t = tid; t < nv * ND; t += T) {
         const int el = t / ND, col = t % ND;
-        emit_column_nh<NN, NQ>(s.sh[el], s.wdv[el], sm.ext.qs[el], col, a.em + (e0 + el) * (ND * ND) + col);
+        emit_column_nh<NN, NQ, TR>(s.sh[el], s.wdv[el], sm.ext.qs[el], col, s.rec[el], a.em);
       }
     } else if constexpr (JAC) {
       for (int t = tid; t < nv * ND; t += T) {
         const int el = t / ND, col = t % ND;
-        emit_column<NN, NQ>(s.sh[el], sm.ext.D[el], col, a.em + (e0 + el) * (ND * ND) + col);
+        emit_column<NN, NQ, TR>(s.sh[el], sm.ext.D[el], col, s.rec[el], a.em);
       }
     }
     __syncthreads();
@@ -427,72 +448,62 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
 }
 
 // One wavefront per mesh node a: rows 4a..4a+3 of the CSR matrix are
-// contiguous (16*deg(a) entries starting at row_ptr[4a]).  The wave reads the
-// row blocks (primal) or column blocks (adjoint) of up to 64 incident
-// elements at a time, all loads independent, adds them into an LDS copy of
-// the node's rows with LDS atomics (fixed instruction and lane order, hence a
-// fixed summation order), then stores the rows once.
+// contiguous (16*deg(a) entries starting at row_ptr[4a]), and so are the
+// row blocks the element pass left for a (records inc_ptr[a] .. inc_ptr[a+1]).
+// The wave streams those blocks, 64 records at a time with the next batch of
+// loads in flight, adds each value into an LDS copy of the node's rows with
+// LDS atomics (fixed instruction and lane order, hence a fixed summation
+// order), then stores the rows once.
 template <int NN>
 __global__ __launch_bounds__(64) void gather_kernel(GatherArgs g) {
-  constexpr int ND = 4 * NN, RB = 4 * ND;
+  constexpr int ND = 4 * NN, RB = 4 * ND, U = 8;
   extern __shared__ double acc[];
   __shared__ double racc[4];
-  __shared__ int32_t sinc[64];
   __shared__ uint16_t spos[64][NN];
   const int lane = threadIdx.x;
   for (int64_t node = blockIdx.x; node < g.num_nodes; node += gridDim.x) {
     const int64_t r0 = g.row_ptr[4 * node];
     const int deg4 = (int)(g.row_ptr[4 * node + 1] - r0);  // 4 * deg(node)
     const int blk = 4 * deg4;
+    const int64_t k0 = g.inc_ptr[node], k1 = g.inc_ptr[node + 1];
     if (g.vals)
       for (int t = lane; t < blk; t += 64) acc[t] = 0.0;
     if (lane < 4) racc[lane] = 0.0;
-    __syncthreads();
-    const int64_t k0 = g.inc_ptr[node], k1 = g.inc_ptr[node + 1];
     for (int64_t kb = k0; kb < k1; kb += 64) {
       const int m = (int)min((int64_t)64, k1 - kb);
-      if (lane < m) sinc[lane] = g.inc[kb + lane];
-      __syncthreads();
-      if (g.vals) {
-        for (int t = lane; t < m * NN; t += 64) {
-          const int32_t pk = sinc[t / NN];
-          spos[t / NN][t % NN] = g.pos[(int64_t)pk * NN + t % NN];
-        }
-        __syncthreads();
-      }
-      for (int v = lane; v < 4 * m; v += 64) {
-        const int32_t pk = sinc[v >> 2];
-        atomicAdd(&racc[v & 3], g.emr[(int64_t)(pk / NN) * ND + 4 * (pk % NN) + (v & 3)]);
-      }
-      if (g.vals) {
-        const int total = m * RB;
-        for (int v0 = 0; v0 < total; v0 += 8 * 64) {
-          double src[8];
-          int slot[8];
+      const double* em = g.vals ? g.em + kb * RB : nullptr;
+      const int total = g.vals ? m * RB : 0;
+      double src[U];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int v = v0 + u * 64 + lane;
-            slot[u] = -1;
-            src[u] = 0.0;
-            if (v < total) {
-              const int j = v / RB, t = v % RB;
-              const int32_t pk = sinc[j];
-              const int64_t e = pk / NN;
-              const int la = pk % NN;
-              const int i = t / ND, c = t % ND;
-              const double* em = g.em + e * (ND * ND);
-              src[u] = g.adjoint ? em[c * ND + 4 * la + i] : em[(4 * la + i) * ND + c];
-              slot[u] = i * deg4 + 4 * (int)spos[j][c >> 2] + (c & 3);
-            }
+      for (int u = 0; u < U; ++u) {
+        const int v = u * 64 + lane;
+        src[u] = v < total ? em[v] : 0.0;
+      }
+      if (g.vals)
+        for (int t = lane; t < m * NN; t += 64) spos[t / NN][t % NN] = g.posi[kb * NN + t];
+      __syncthreads();  // acc / racc cleared, spos staged
+      for (int v = lane; v < 4 * m; v += 64) atomicAdd(&racc[v & 3], g.emr[kb * 4 + v]);
+      for (int v0 = 0; v0 < total; v0 += U * 64) {
+        double nxt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int v = v0 + (U + u) * 64 + lane;
+          nxt[u] = v < total ? em[v] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int v = v0 + u * 64 + lane;
+          if (v < total) {
+            const int j = v / RB, t = v % RB;
+            const int i = t / ND, c = t % ND;
+            atomicAdd(&acc[i * deg4 + 4 * (int)spos[j][c >> 2] + (c & 3)], src[u]);
           }
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (slot[u] >= 0) atomicAdd(&acc[slot[u]], src[u]);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) src[u] = nxt[u];
       }
-      __syncthreads();  // sinc is rewritten by the next chunk
+      __syncthreads();  // spos is rewritten by the next chunk; acc complete
     }
-    __syncthreads();
     if (g.R && lane < 4) {
       const int64_t row = 4 * node + lane;
       g.R[row] = (g.overwrite_R ? 0.0 : g.R[row]) + racc[lane];
@@ -546,7 +557,7 @@ hipError_t launch3(int kind, const AsmArgs& a, int blocks, hipStream_t s) {
   switch (kind) {
     case ASM_RESIDUAL: return launch4<NN, NQ, MODEL, ASM_RESIDUAL>(a, blocks, s);
     case ASM_JACOBIAN: return launch4<NN, NQ, MODEL, ASM_JACOBIAN>(a, blocks, s);
-    case ASM_ADJOINT: return launch4<NN, NQ, MODEL, ASM_JACOBIAN>(a, blocks, s);
+    case ASM_ADJOINT: return launch4<NN, NQ, MODEL, ASM_ADJOINT>(a, blocks, s);
     case ASM_LOCALIZE: return launch4<NN, NQ, MODEL, ASM_LOCALIZE>(a, blocks, s);
     default: return hipErrorInvalidValue;
   }

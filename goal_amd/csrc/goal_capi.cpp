@@ -160,8 +160,8 @@ struct goal_disc_s {
   int max_degree = 0;
   int64_t num_dofs = 0, nnz = 0;
   DevBuf<double> coords, x;
-  DevBuf<int32_t> conn, elem_set, cols, vertex_nodes, inc;
-  DevBuf<uint16_t> pos;
+  DevBuf<int32_t> conn, elem_set, cols, vertex_nodes, inc, rec;
+  DevBuf<uint16_t> pos, posi;
   DevBuf<int64_t> row_ptr, inc_ptr;
   std::map<std::string, std::unique_ptr<DevBuf<int32_t>>> dev_node_sets;
   DevBuf<double> dbc_values;
@@ -217,6 +217,8 @@ void disc_build(goal_disc d) {
   d->pos.upload(g.pos.data(), (int64_t)g.pos.size());
   d->inc_ptr.upload(g.inc_ptr.data(), (int64_t)g.inc_ptr.size());
   d->inc.upload(g.inc.data(), (int64_t)g.inc.size());
+  d->rec.upload(g.rec.data(), (int64_t)g.rec.size());
+  d->posi.upload(g.posi.data(), (int64_t)g.posi.size());
   DevBuf<int64_t> adj_ptr;
   DevBuf<int32_t> adj;
   adj_ptr.upload(g.node_adj_ptr.data(), (int64_t)g.node_adj_ptr.size());
@@ -306,6 +308,7 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
   const int64_t nd = 4 * (int64_t)d->mesh.nodes_per_elem;
   if (m->emr.n < d->mesh.num_elems * nd) m->emr.alloc(d->mesh.num_elems * nd);
   if (want_vals && m->em.n < d->mesh.num_elems * nd * nd) m->em.alloc(d->mesh.num_elems * nd * nd);
+  a.rec = d->rec.p;
   a.emr = m->emr.p;
   a.em = want_vals ? m->em.p : nullptr;
   HIPCHK(goal::launch_element(kind, d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
@@ -320,15 +323,13 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
   goal::GatherArgs g{};
   g.num_nodes = d->mesh.num_nodes;
   g.inc_ptr = d->inc_ptr.p;
-  g.inc = d->inc.p;
-  g.pos = d->pos.p;
+  g.posi = d->posi.p;
   g.row_ptr = d->row_ptr.p;
   g.emr = m->emr.p;
   g.em = want_vals ? m->em.p : nullptr;
   g.R = s->R.p;
   g.vals = want_vals ? s->vals.p : nullptr;
   g.max_degree = d->max_degree;
-  g.adjoint = kind == goal::ASM_ADJOINT ? 1 : 0;
   g.overwrite_R = s->zero_R ? 1 : 0;
   g.overwrite_vals = (want_vals && s->zero_vals) ? 1 : 0;
   HIPCHK(goal::launch_gather(d->mesh.order, g, gather_blocks(d), stream()));

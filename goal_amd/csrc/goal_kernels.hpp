@@ -33,8 +33,13 @@ struct AsmArgs {
   const double* x;          // [num_dofs] u,p interleaved per node
   const int32_t* elem_set;  // [ne]
   const MatParams* mats;    // [num_sets]
-  double* emr;              // [ne][4*npe] element residuals
-  double* em;               // [ne][4*npe][4*npe] element matrices (row-major) or null
+  const int32_t* rec;       // [ne][npe] incidence record of (element, local node)
+  // Element residual rows and matrix row blocks, node-major: the block of
+  // (element e, local node a) is stored at record k = rec[e][a], so the blocks
+  // a node sums are contiguous.  emr[k][4], em[k][4][4*npe] (for
+  // ASM_ADJOINT the row blocks of the transposed element matrix).
+  double* emr;
+  double* em;               // null: residual only
   // per quadrature point history, structure of arrays: comp * (ne*nq) + e*nq + q
   const double* Fp_old;
   const double* eqps_old;
@@ -54,16 +59,14 @@ struct AsmArgs {
 // Node-owner gather of element residuals / matrices into R and the CSR values.
 struct GatherArgs {
   int64_t num_nodes;
-  const int64_t* inc_ptr;   // [num_nodes + 1] node -> incident elements
-  const int32_t* inc;       // packed e * npe + local node
-  const uint16_t* pos;      // [ne][npe][npe] position of node b in adj(node a)
+  const int64_t* inc_ptr;   // [num_nodes + 1] node -> its incidence records
+  const uint16_t* posi;     // [records][npe] position of each element node in adj(node)
   const int64_t* row_ptr;   // [num_dofs + 1]
-  const double* emr;
+  const double* emr;        // node-major, see AsmArgs
   const double* em;         // null: residual only
   double* R;                // null: skip
   double* vals;             // null: skip
   int max_degree;
-  int adjoint;              // 1: scatter the transpose (dRdu^T)
   int overwrite_R;          // 1: R = sum (R was pending zero), 0: R += sum
   int overwrite_vals;
 };

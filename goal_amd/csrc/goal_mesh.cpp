@@ -252,6 +252,14 @@ HostGraph build_graph(const HostMesh& m, int num_threads) {
       }
     }
   });
+  const int64_t ninc = (int64_t)g.inc.size();
+  g.rec.resize(ninc);
+  for (int64_t k = 0; k < ninc; ++k) g.rec[g.inc[k]] = (int32_t)k;
+  g.posi.resize(ninc * npe);
+  parallel_for(ninc, num_threads, [&](int, int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; ++k)
+      std::copy_n(&g.pos[(int64_t)g.inc[k] * npe], npe, &g.posi[k * npe]);
+  });
   return g;
 }
 

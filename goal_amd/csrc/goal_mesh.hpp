@@ -57,6 +57,11 @@ struct HostGraph {
   // ascending element order.
   std::vector<int64_t> inc_ptr;  // [num_nodes + 1]
   std::vector<int32_t> inc;
+  // Incidence record of (element e, local node a): inc[rec[e * npe + a]] ==
+  // e * npe + a.  [num_elems][npe]
+  std::vector<int32_t> rec;
+  // pos rows in incidence order: posi[k][b] = pos[inc[k]][b].  [inc.size()][npe]
+  std::vector<uint16_t> posi;
 };
 
 // Kuhn (Freudenthal) box mesh: nx*ny*nz hexahedra on [0,lx]x[0,ly]x[0,lz],
