@@ -28,6 +28,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 B_ELEM_P2 = 13576  # algorithmic bytes per P2/P2 element, residual + Jacobian pass
 B_ELEM_P1 = 2416
+NCOPY = 1 << 28  # doubles per buffer of the in-run copy kernel (2 GiB read + 2 GiB written)
+ASSEMBLY_KERNELS = ("element_kernel", "gather_kernel")
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profile_tag(n: int, order: int) -> str:
+    return f"assembly_p{order}_nh_cube{n}"
 
 
 def log(msg: str) -> None:
@@ -73,6 +80,98 @@ def cpu_baseline(n_sample: int, threads: int, order: int, quad: int, reps: int =
     return m.num_elems / med, m.num_elems, med
 
 
+def _read_csv(path):
+    import csv
+
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find(dirpath, suffix):
+    for root, _, files in os.walk(dirpath):
+        for fn in files:
+            if fn.endswith(suffix):
+                return os.path.join(root, fn)
+    raise FileNotFoundError(f"no *{suffix} under {dirpath}")
+
+
+def _short(name: str) -> str:
+    for k in ASSEMBLY_KERNELS + ("copy_kernel", "fill_kernel", "expand_csr_kernel"):
+        if k in name:
+            return k
+    return name
+
+
+def summarize(kt_dir, fetch_dir, write_dir, tag, commands):
+    """Turn rocprofv3 CSV output (kernel trace + stats, FETCH_SIZE pass,
+    WRITE_SIZE pass) into the text summaries kept under profiles/."""
+    stats = _read_csv(_find(kt_dir, "kernel_stats.csv"))
+    per = {}
+    for which, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        for r in _read_csv(_find(d, "counter_collection.csv")):
+            if r["Counter_Name"] != which:
+                continue
+            k = _short(r["Kernel_Name"])
+            per.setdefault(k, {}).setdefault(which, []).append(float(r["Counter_Value"]))
+    mean = {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in per.items()}
+    # calibration: the copy kernel reads and writes 8 * NCOPY bytes per launch
+    cbytes = 8.0 * NCOPY
+    cal_fetch = cbytes / (mean["copy_kernel"]["FETCH_SIZE"] * 1024.0)
+    cal_write = cbytes / (mean["copy_kernel"]["WRITE_SIZE"] * 1024.0)
+    kern = {}
+    for r in stats:
+        k = _short(r["Name"])
+        if k in ASSEMBLY_KERNELS or k == "copy_kernel":
+            kern[k] = {"calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"])}
+    out = {"tag": tag, "commands": commands, "calibration": {"fetch": cal_fetch, "write": cal_write},
+           "kernels": {}}
+    for k in ASSEMBLY_KERNELS:
+        fk, wk = mean[k]["FETCH_SIZE"], mean[k]["WRITE_SIZE"]
+        out["kernels"][k] = {"avg_ms": kern[k]["avg_ns"] / 1e6, "fetch_kib": fk, "write_kib": wk,
+                             "read_bytes": fk * 1024.0 * cal_fetch, "write_bytes": wk * 1024.0 * cal_write}
+    out["traffic_bytes"] = sum(v["read_bytes"] + v["write_bytes"] for v in out["kernels"].values())
+    with open(os.path.join(PROFILES, f"{tag}_traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    lines = [f"# {tag}: kernel trace and HBM traffic", "", "Commands (repository root on the GPU host):", ""]
+    lines += [f"    {c}" for c in commands]
+    lines += ["", "## rocprofv3 --kernel-trace --stats", "", "| kernel | calls | average ms | total ms | % |",
+              "|---|---|---|---|---|"]
+    for r in stats:
+        lines.append(f"| `{r['Name'][:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} |")
+    lines += ["", "## FETCH_SIZE / WRITE_SIZE (KiB per dispatch, mean)", "",
+              f"Calibration: the copy kernel moves {cbytes / 2**30:.0f} GiB each way per launch; the counters read "
+              f"{mean['copy_kernel']['FETCH_SIZE'] / 2**20:.3f} GiB fetched and "
+              f"{mean['copy_kernel']['WRITE_SIZE'] / 2**20:.3f} GiB written, so FETCH_SIZE is scaled by "
+              f"{cal_fetch:.4f} and WRITE_SIZE by {cal_write:.4f}.", "",
+              "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB (scaled) | written GB (scaled) |",
+              "|---|---|---|---|---|"]
+    for k in ASSEMBLY_KERNELS + ("copy_kernel",):
+        fk, wk = mean[k]["FETCH_SIZE"], mean[k]["WRITE_SIZE"]
+        lines.append(f"| {k} | {fk:.0f} | {wk:.0f} | {fk * 1024 * cal_fetch / 1e9:.3f} | "
+                     f"{wk * 1024 * cal_write / 1e9:.3f} |")
+    lines += ["", f"HBM traffic per assembly (element + gather): {out['traffic_bytes'] / 1e9:.3f} GB", ""]
+    with open(os.path.join(PROFILES, f"{tag}.md"), "w") as f:
+        f.write("\n".join(lines))
+    print(json.dumps({"tag": tag, "traffic_GB": out["traffic_bytes"] / 1e9,
+                      "avg_ms": {k: v["avg_ms"] for k, v in out["kernels"].items()}}))
+    return 0
+
+
+def load_traffic(tag, element_ms, gather_ms):
+    """HBM bytes per assembly from the committed profile of this workload, if
+    its kernel times match this run's within 15% (otherwise it is stale)."""
+    path = os.path.join(PROFILES, f"{tag}_traffic.json")
+    if not os.path.exists(path) or element_ms is None:
+        return None, "no profile"
+    with open(path) as f:
+        prof = json.load(f)
+    ke, kg = prof["kernels"]["element_kernel"]["avg_ms"], prof["kernels"]["gather_kernel"]["avg_ms"]
+    if abs(ke - element_ms) > 0.15 * element_ms or abs(kg - gather_ms) > 0.15 * gather_ms:
+        return None, f"stale profile (element {ke:.2f} ms, gather {kg:.2f} ms)"
+    return prof["traffic_bytes"], os.path.relpath(path, ROOT)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,15 +181,27 @@ def main() -> int:
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--blocks", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=16, help="hexahedra per side of the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="hexahedra per side of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--summarize", nargs=3, metavar=("KT_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/ summaries from rocprofv3 CSV output of this benchmark and exit")
     args = ap.parse_args()
+
+    if args.summarize:
+        base = f"python3 bench.py --n {args.n} --order {args.order}"
+        cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d {args.summarize[0]} -o run -- "
+                f"{base} --steps 5 --warmup 1 --no-cpu-baseline",
+                f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {args.summarize[1]} -o run -- "
+                f"{base} --steps 2 --warmup 1 --no-cpu-baseline",
+                f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {args.summarize[2]} -o run -- "
+                f"{base} --steps 2 --warmup 1 --no-cpu-baseline"]
+        return summarize(*args.summarize, profile_tag(args.n, args.order), cmds)
 
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: this build has no multi-GPU path "
-            "(the assembly would shard by mesh partition; see DESIGN.md). Use --gpus 1.",
+            "bench.py measures one GPU: this build has no multi-GPU path (the assembly would shard by "
+            "mesh partition, one part per GPU, with an exchange of interface rows). Use --gpus 1.",
             file=sys.stderr,
         )
         return 2
@@ -143,12 +254,23 @@ def main() -> int:
     elems_per_s = disc.num_elems / (ms_per_step / 1e3)
     log(f"{K} steps: {ms_per_step:.3f} ms/step, assembly kernel {kernel_ms:.3f} ms, {elems_per_s / 1e6:.2f} M elem/s")
 
+    # element pass / gather split, from events the library records around each kernel
+    mech.set_option("timing", 1)
+    splits = []
+    for _ in range(5):
+        step()
+        splits.append(mech.timing())
+    mech.set_option("timing", 0)
+    element_ms = statistics.mean(e for e, _ in splits)
+    gather_ms = statistics.mean(gg for _, gg in splits)
+    log(f"element pass {element_ms:.3f} ms, gather {gather_ms:.3f} ms")
+
     # sanity: the measured state is a real assembly (finite, nonzero)
     R = si.R()
     assert np.isfinite(R).all() and np.abs(R).max() > 0
 
     # in-run STREAM-style copy bandwidth (read + write bytes / time)
-    ncopy = 1 << 28  # 2 GiB per buffer
+    ncopy = NCOPY
     a = g.DeviceArray(ncopy)
     b = g.DeviceArray(ncopy)
     g.device_copy(b, a)
@@ -164,13 +286,14 @@ def main() -> int:
 
     b_elem = B_ELEM_P2 if order == 2 else B_ELEM_P1
     achieved = b_elem * disc.num_elems / (kernel_ms * 1e-3) / 1e9 if kernel_ms else None
+    traffic, traffic_src = load_traffic(profile_tag(n, order), element_ms, gather_ms)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1) if achieved else None,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-        "traffic": None,
+        "traffic": round(traffic) if traffic else None,
     }
 
     cpu = None
@@ -214,6 +337,11 @@ def main() -> int:
         "cpu_baseline": cpu,
         "detail": {
             "kernel_ms": round(kernel_ms, 4) if kernel_ms else None,
+            "element_kernel_ms": round(element_ms, 4),
+            "gather_kernel_ms": round(gather_ms, 4),
+            "roofline_unit": "one assembly launch = element_kernel + gather_kernel on one stream, "
+                             "HIP events around both; B_elem x elements / its duration",
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_elem": b_elem,
             "copy_kernel_GBs": round(copy_gbs, 1),
             "device": name,

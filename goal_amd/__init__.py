@@ -125,6 +125,7 @@ _sig("goal_solinfo_dot_R", _i32, _p, _p, _dp)
 _sig("goal_mech_create", _i32, _p, _p, _i32, _p, _i32, _i32, C.POINTER(_p))
 _sig("goal_mech_destroy", _i32, _p)
 _sig("goal_mech_set_option", _i32, _p, C.c_char_p, _i64)
+_sig("goal_mech_timing", _i32, _p, C.POINTER(C.c_float), C.POINTER(C.c_float))
 _sig("goal_mech_status", _i32, _p, C.POINTER(C.c_int))
 _sig("goal_asm_residual", _i32, _p, _p, _i32)
 _sig("goal_asm_jacobian", _i32, _p, _p, _i32, _i32)
@@ -466,6 +467,12 @@ class Mechanics:
 
     def set_option(self, key: str, value: int) -> None:
         check(lib.goal_mech_set_option(self.h, key.encode(), int(value)))
+
+    def timing(self) -> tuple[float, float]:
+        """(element pass ms, gather ms) of the last assembly; needs set_option("timing", 1)."""
+        a, b = C.c_float(0.0), C.c_float(0.0)
+        check(lib.goal_mech_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def status(self) -> int:
         f = C.c_int(0)

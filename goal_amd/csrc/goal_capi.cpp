@@ -191,6 +191,12 @@ struct goal_mech_s {
   DevBuf<int32_t> status;
   DevBuf<double> elem_value;
   DevBuf<double> emr, em;  // element residuals / matrices between the two passes
+  bool timing = false, timed = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  ~goal_mech_s() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 struct goal_nested_s {
@@ -311,7 +317,9 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
   a.rec = d->rec.p;
   a.emr = m->emr.p;
   a.em = want_vals ? m->em.p : nullptr;
+  if (m->timing) HIPCHK(hipEventRecord(m->ev[0], stream()));
   HIPCHK(goal::launch_element(kind, d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
+  if (m->timing) HIPCHK(hipEventRecord(m->ev[1], stream()));
   if (s->zero_dMdu) {
     HIPCHK(goal::launch_fill(s->dMdu.p, s->dMdu.n, 0.0, stream()));
     s->zero_dMdu = false;
@@ -333,6 +341,10 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
   g.overwrite_R = s->zero_R ? 1 : 0;
   g.overwrite_vals = (want_vals && s->zero_vals) ? 1 : 0;
   HIPCHK(goal::launch_gather(d->mesh.order, g, gather_blocks(d), stream()));
+  if (m->timing) {
+    HIPCHK(hipEventRecord(m->ev[2], stream()));
+    m->timed = true;
+  }
   s->zero_R = false;
   if (want_vals) s->zero_vals = false;
 }
@@ -931,9 +943,26 @@ int goal_mech_set_option(goal_mech m, const char* key, int64_t value) {
       m->check = value ? 1 : 0;
     } else if (k == "stabilization") {
       m->stab = value ? 1 : 0;
+    } else if (k == "timing") {
+      if (value)
+        for (auto& e : m->ev)
+          if (!e) HIPCHK(hipEventCreate(&e));
+      m->timing = value != 0;
+      m->timed = false;
     } else {
       return set_error(GOAL_ERR_INVALID, "unknown option: %s", key);
     }
+    return GOAL_OK;
+  });
+}
+
+int goal_mech_timing(goal_mech m, float* element_ms, float* gather_ms) {
+  return guard([&]() -> int {
+    require(m && element_ms && gather_ms, "bad arguments");
+    require(m->timing && m->timed, "goal_mech_timing: no timed assembly (set the \"timing\" option first)");
+    HIPCHK(hipEventSynchronize(m->ev[2]));
+    HIPCHK(hipEventElapsedTime(element_ms, m->ev[0], m->ev[1]));
+    HIPCHK(hipEventElapsedTime(gather_ms, m->ev[1], m->ev[2]));
     return GOAL_OK;
   });
 }

@@ -164,8 +164,15 @@ int goal_mech_create(goal_disc d, goal_states st, int num_sets, const goal_mater
 int goal_mech_destroy(goal_mech m);
 /* Options: "blocks" (workgroups per launch, 0 = auto), "xcd_remap" (0/1),
  * "check" (1: synchronise after each assembly and report J2 failures),
- * "stabilization" (0/1, overrides the value given at creation). */
+ * "stabilization" (0/1, overrides the value given at creation),
+ * "timing" (1: record device events around the element and gather kernels of
+ * each assembly, read back with goal_mech_timing). */
 int goal_mech_set_option(goal_mech m, const char* key, int64_t value);
+/* Device time of the last assembly's element pass and node gather, in ms
+ * (waits for it).  Needs the "timing" option.  No counterpart in the
+ * reference (goal::time() brackets whole phases on the host,
+ * src/goal_primal.cpp:76-108); it is for profiling callers. */
+int goal_mech_timing(goal_mech m, float* element_ms, float* gather_ms);
 /* Accumulated kernel status bits since the last call (1: J2 return map failed). */
 int goal_mech_status(goal_mech m, int* flags);
 
