@@ -174,6 +174,33 @@ def test_patch_test_affine_field():
         assert np.allclose(st.sigma.T, sig.ravel()[None, :], rtol=0, atol=1e-10)
 
 
+def test_elastic_is_linearized_neohookean():
+    """Without stabilization the Elastic model (src/goal_elastic.cpp:61-70 with
+    the small-strain PResidual) is the neo-Hookean chain linearized at u = 0,
+    p = 0: momentum rows agree, pressure rows are kappa times the neo-Hookean
+    ones (p - (2mu/3 + lambda) tr H = kappa (p/kappa - tr H)).  Exact for the
+    Jacobian at the reference state, first order for the residual."""
+    m = box(2, 2)
+    kappa = NH["E"] / (3 * (1 - 2 * NH["nu"]))
+    rows = np.repeat(np.arange(m.num_dofs), np.diff(m.row_ptr))
+    scale = np.where(rows % 4 == 3, kappa, 1.0)
+    for quad in (1, 2):
+        x0 = np.zeros(m.num_dofs)
+        _, VE = O.assemble(O.K_JACOBIAN, m, 0, [mat_row(ELASTIC)], x0, O.States(m, 0, quad), quad_order=quad,
+                           stabilization=False)
+        _, VN = O.assemble(O.K_JACOBIAN, m, 1, [mat_row(NH)], x0, O.States(m, 1, quad), quad_order=quad,
+                           stabilization=False)
+        assert np.abs(VE - scale * VN).max() <= 1e-12 * np.abs(VE).max()
+        x = random_state(m.num_nodes, 5, scale_u=1e-6, scale_p=1e-3)
+        RE, _ = O.assemble(O.K_RESIDUAL, m, 0, [mat_row(ELASTIC)], x, O.States(m, 0, quad), quad_order=quad,
+                           stabilization=False)
+        RN, _ = O.assemble(O.K_RESIDUAL, m, 1, [mat_row(NH)], x, O.States(m, 1, quad), quad_order=quad,
+                           stabilization=False)
+        for i in range(3):
+            assert np.linalg.norm(RE[i::4] - RN[i::4]) <= 1e-4 * np.linalg.norm(RN[i::4])
+        assert np.linalg.norm(RE[3::4] - kappa * RN[3::4]) <= 1e-4 * np.linalg.norm(RE[3::4])
+
+
 def test_j2_elastic_branch_equals_neohookean():
     m = box(2, 1)
     x = random_state(m.num_nodes, 4)
