@@ -33,8 +33,8 @@ ASSEMBLY_KERNELS = ("element_kernel", "gather_kernel")
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def profile_tag(n: int, order: int) -> str:
-    return f"assembly_p{order}_nh_cube{n}"
+def profile_tag(n: int, order: int, tangent: int = 64) -> str:
+    return f"assembly_p{order}_nh_cube{n}" + ("_f32tangent" if tangent == 32 else "")
 
 
 def log(msg: str) -> None:
@@ -184,19 +184,21 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=32, help="hexahedra per side of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tangent", type=int, default=64, choices=(64, 32),
+                    help="element-matrix precision of the measured path (the residual is FP64 either way)")
     ap.add_argument("--summarize", nargs=3, metavar=("KT_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/ summaries from rocprofv3 CSV output of this benchmark and exit")
     args = ap.parse_args()
 
     if args.summarize:
-        base = f"python3 bench.py --n {args.n} --order {args.order}"
+        base = f"python3 bench.py --n {args.n} --order {args.order} --tangent {args.tangent}"
         cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d {args.summarize[0]} -o run -- "
                 f"{base} --steps 5 --warmup 1 --no-cpu-baseline",
                 f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {args.summarize[1]} -o run -- "
                 f"{base} --steps 2 --warmup 1 --no-cpu-baseline",
                 f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {args.summarize[2]} -o run -- "
                 f"{base} --steps 2 --warmup 1 --no-cpu-baseline"]
-        return summarize(*args.summarize, profile_tag(args.n, args.order), cmds)
+        return summarize(*args.summarize, profile_tag(args.n, args.order, args.tangent), cmds)
 
     if args.gpus != 1:
         print(
@@ -225,6 +227,7 @@ def main() -> int:
     if args.blocks:
         mech.set_option("blocks", args.blocks)
     mech.set_option("xcd_remap", args.xcd_remap)
+    mech.set_option("tangent", args.tangent)
     coords, _, _ = disc.mesh()
     disc.set_soln(smooth_state(coords))
     si = g.SolInfo(disc)
@@ -265,6 +268,29 @@ def main() -> int:
     gather_ms = statistics.mean(gg for _, gg in splits)
     log(f"element pass {element_ms:.3f} ms, gather {gather_ms:.3f} ms")
 
+    # the other element-matrix precision, same mesh and iterate, for comparison
+    other = 32 if args.tangent == 64 else 64
+    mech.set_option("tangent", other)
+    for _ in range(args.warmup):
+        step()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        step()
+    g.sync()
+    other_ms = (time.perf_counter() - t0) * 1e3 / K
+    mech.set_option("timing", 1)
+    osplit = []
+    for _ in range(5):
+        step()
+        osplit.append(mech.timing())
+    mech.set_option("timing", 0)
+    mech.set_option("tangent", args.tangent)
+    log(f"tangent {other}: {other_ms:.3f} ms/step (element {statistics.mean(e for e, _ in osplit):.3f} ms, "
+        f"gather {statistics.mean(gg for _, gg in osplit):.3f} ms)")
+    step()
+    g.sync()
+
     # sanity: the measured state is a real assembly (finite, nonzero)
     R = si.R()
     assert np.isfinite(R).all() and np.abs(R).max() > 0
@@ -286,7 +312,7 @@ def main() -> int:
 
     b_elem = B_ELEM_P2 if order == 2 else B_ELEM_P1
     achieved = b_elem * disc.num_elems / (kernel_ms * 1e-3) / 1e9 if kernel_ms else None
-    traffic, traffic_src = load_traffic(profile_tag(n, order), element_ms, gather_ms)
+    traffic, traffic_src = load_traffic(profile_tag(n, order, args.tangent), element_ms, gather_ms)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1) if achieved else None,
@@ -323,7 +349,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.tangent == 64 else "f64 residual, f32 element matrices",
         "data": "synthetic (Kuhn-tet cube mesh, smooth synthetic displacement/pressure iterate)",
         "config": {
             "workload": f"cube {n}^3 hex -> {disc.num_elems} P2 tets, {disc.num_dofs} dofs, nnz {disc.nnz}, "
@@ -342,6 +368,11 @@ def main() -> int:
             "roofline_unit": "one assembly launch = element_kernel + gather_kernel on one stream, "
                              "HIP events around both; B_elem x elements / its duration",
             "traffic_source": traffic_src,
+            "tangent_bits": args.tangent,
+            f"tangent{other}": {"ms_per_step": round(other_ms, 4),
+                                "elem_per_s": round(disc.num_elems / (other_ms * 1e-3), 1),
+                                "element_kernel_ms": round(statistics.mean(e for e, _ in osplit), 4),
+                                "gather_kernel_ms": round(statistics.mean(gg for _, gg in osplit), 4)},
             "algorithmic_bytes_per_elem": b_elem,
             "copy_kernel_GBs": round(copy_gbs, 1),
             "device": name,

@@ -145,28 +145,30 @@ __device__ inline int64_t group_of(int64_t it, int64_t ngroups, int remap) {
 // Store column col = (b, kc) of an element matrix: into the row blocks of the
 // element's nodes (rows (a, i) at record rec[a]), or, transposed, as row kc of
 // node b's block.
-template <int NN, bool TR>
-__device__ inline void store_column(const double (*acc)[4], int col, const int32_t* rec, double* em) {
+template <int NN, bool TR, typename T>
+__device__ inline void store_column(const T (*acc)[4], int col, const int32_t* rec, T* em) {
   constexpr int ND = 4 * NN;
   if constexpr (!TR) {
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
-      double* o = em + (int64_t)rec[n] * (4 * ND) + col;
+      T* o = em + (int64_t)rec[n] * (4 * ND) + col;
       for (int i = 0; i < 4; ++i) o[i * ND] = acc[n][i];
     }
   } else {
-    double* o = em + (int64_t)rec[col >> 2] * (4 * ND) + (col & 3) * ND;
+    T* o = em + (int64_t)rec[col >> 2] * (4 * ND) + (col & 3) * ND;
 #pragma unroll
     for (int n = 0; n < NN; ++n)
       for (int i = 0; i < 4; ++i) o[4 * n + i] = acc[n][i];
   }
 }
 
-template <int NN, int NQ, bool TR>
+// T: the element-matrix type (double; float for the FP32 tangent, which forms
+// and accumulates K in FP32 from the FP64 point tangent).
+template <int NN, int NQ, bool TR, typename T>
 __device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[13][13], int col,
-                                   const int32_t* rec, double* em) {
+                                   const int32_t* rec, T* em) {
   const int b = col >> 2, kc = col & 3;
-  double acc[NN][4];
+  T acc[NN][4];
 #pragma unroll
   for (int n = 0; n < NN; ++n)
     for (int i = 0; i < 4; ++i) acc[n][i] = 0.0;
@@ -174,33 +176,33 @@ __device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[
   for (int q = 0; q < NQ; ++q) {
     const double* shb = sh[q][b];
     const double sb0 = shb[0], sb1 = shb[1], sb2 = shb[2], sb3 = shb[3];
-    double C[13];
+    T C[13];
 #pragma unroll
     for (int r = 0; r < 13; ++r) {
       const double* Dr = D[q][r];
-      C[r] = (kc < 3) ? Dr[3 * kc] * sb1 + Dr[3 * kc + 1] * sb2 + Dr[3 * kc + 2] * sb3
-                      : Dr[9] * sb0 + Dr[10] * sb1 + Dr[11] * sb2 + Dr[12] * sb3;
+      C[r] = (T)((kc < 3) ? Dr[3 * kc] * sb1 + Dr[3 * kc + 1] * sb2 + Dr[3 * kc + 2] * sb3
+                          : Dr[9] * sb0 + Dr[10] * sb1 + Dr[11] * sb2 + Dr[12] * sb3);
     }
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
       const double* sa = sh[q][n];
-      const double s0 = sa[0], s1 = sa[1], s2 = sa[2], s3 = sa[3];
+      const T s0 = (T)sa[0], s1 = (T)sa[1], s2 = (T)sa[2], s3 = (T)sa[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) acc[n][i] += C[3 * i] * s1 + C[3 * i + 1] * s2 + C[3 * i + 2] * s3;
       acc[n][3] += C[9] * s0 + C[10] * s1 + C[11] * s2 + C[12] * s3;
     }
   }
-  store_column<NN, TR>(acc, col, rec, em);
+  store_column<NN, TR, T>(acc, col, rec, em);
 }
 
 // neo-Hookean column (b, kc): C_q from the closed-form tangent contracted
 // with node b's shape function (dev::nh_direction), then the rows as above.
-template <int NN, int NQ, bool TR>
+template <int NN, int NQ, bool TR, typename T>
 __device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* wdv,
                                       const QPStore<MODEL_NEOHOOKEAN>* qs, int col, const int32_t* rec,
-                                      double* em) {
+                                      T* em) {
   const int b = col >> 2, kc = col & 3;
-  double acc[NN][4];
+  T acc[NN][4];
 #pragma unroll
   for (int n = 0; n < NN; ++n)
     for (int i = 0; i < 4; ++i) acc[n][i] = 0.0;
@@ -208,22 +210,22 @@ __device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* w
   for (int q = 0; q < NQ; ++q) {
     const double w = wdv[q];
     const double* shb = sh[q][b];
-    const double G[3] = {w * shb[1], w * shb[2], w * shb[3]};
-    double C[13];
-    dev::nh_direction(qs[q].b, kc, w * shb[0], G, C);
+    const T G[3] = {(T)(w * shb[1]), (T)(w * shb[2]), (T)(w * shb[3])};
+    T C[13];
+    dev::nh_direction<T>(qs[q].b, kc, (T)(w * shb[0]), G, C);
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
       const double* sa = sh[q][n];
-      const double s0 = sa[0], s1 = sa[1], s2 = sa[2], s3 = sa[3];
+      const T s0 = (T)sa[0], s1 = (T)sa[1], s2 = (T)sa[2], s3 = (T)sa[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) acc[n][i] += C[3 * i] * s1 + C[3 * i + 1] * s2 + C[3 * i + 2] * s3;
       acc[n][3] += C[9] * s0 + C[10] * s1 + C[11] * s2 + C[12] * s3;
     }
   }
-  store_column<NN, TR>(acc, col, rec, em);
+  store_column<NN, TR, T>(acc, col, rec, em);
 }
 
-template <int NN, int NQ, int MODEL, int KIND>
+template <int NN, int NQ, int MODEL, int KIND, typename TE>
 __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>::W)) void element_kernel(AsmArgs a) {
   using Lay = Layout<NN, NQ, MODEL>;
   using M = typename ModelOf<MODEL>::type;
@@ -432,100 +434,170 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
 
     // ---- E: Jacobian, one (element, column) per lane
     constexpr bool TR = (KIND == ASM_ADJOINT);
+    TE* em;
+    if constexpr (std::is_same<TE, float>::value) em = a.em32;
+    else em = a.em;
     if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
       for (int t = tid; t < nv * ND; t += T) {
         const int el = t / ND, col = t % ND;
-        emit_column_nh<NN, NQ, TR>(s.sh[el], s.wdv[el], sm.ext.qs[el], col, s.rec[el], a.em);
+        emit_column_nh<NN, NQ, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], col, s.rec[el], em);
       }
     } else if constexpr (JAC) {
       for (int t = tid; t < nv * ND; t += T) {
         const int el = t / ND, col = t % ND;
-        emit_column<NN, NQ, TR>(s.sh[el], sm.ext.D[el], col, s.rec[el], a.em);
+        emit_column<NN, NQ, TR, TE>(s.sh[el], sm.ext.D[el], col, s.rec[el], em);
       }
     }
     __syncthreads();
   }
 }
 
-// One wavefront per mesh node a: rows 4a..4a+3 of the CSR matrix are
-// contiguous (16*deg(a) entries starting at row_ptr[4a]), and so are the
-// row blocks the element pass left for a (records inc_ptr[a] .. inc_ptr[a+1]).
-// The wave streams those blocks, 64 records at a time with the next batch of
-// loads in flight, adds each value into an LDS copy of the node's rows with
-// LDS atomics (fixed instruction and lane order, hence a fixed summation
-// order), then stores the rows once.
-template <int NN>
-__global__ __launch_bounds__(64) void gather_kernel(GatherArgs g) {
-  constexpr int ND = 4 * NN, RB = 4 * ND, U = 8;
+// Node-range gather.  Rows 4a..4a+3 of the CSR matrix are contiguous for
+// every node a, and so are the element row blocks the element pass left for a
+// (records inc_ptr[a] .. inc_ptr[a+1]); for a run of consecutive nodes both
+// stay contiguous.  A workgroup takes one run (planned on the host so its CSR
+// values fit the LDS budget `cap`), clears an LDS copy of those values, streams
+// the run's blocks with U loads per lane in flight and the next batch
+// prefetched, adds each value at its slot with LDS atomics, then stores the
+// rows once.  Wave i of the four handles row i (equation i) of every node, so
+// no two waves touch the same slot and each slot is summed in one wave's fixed
+// instruction and lane order: results are bitwise reproducible.  No atomics
+// to HBM, no zero fill of the CSR values.
+#ifndef GOAL_GATHER_U64
+#define GOAL_GATHER_U64 2
+#endif
+#ifndef GOAL_GATHER_U32
+#define GOAL_GATHER_U32 4
+#endif
+constexpr int kGatherThreads = 256;  // four waves: one per equation
+constexpr int kGatherRecords = 256;  // incidence records staged per chunk
+constexpr int kGatherNodes = 64;     // nodes per run, at most
+
+template <int NN, typename TE>
+__global__ __launch_bounds__(kGatherThreads) void gather_kernel(GatherArgs g) {
+  constexpr int ND = 4 * NN, RB = 4 * ND, NT = kGatherThreads, RC = kGatherRecords, NC = kGatherNodes;
+  constexpr int U = sizeof(TE) == 8 ? GOAL_GATHER_U64 : GOAL_GATHER_U32;  // loads per lane per batch
   extern __shared__ double acc[];
-  __shared__ double racc[4];
-  __shared__ uint16_t spos[64][NN];
-  const int lane = threadIdx.x;
-  for (int64_t node = blockIdx.x; node < g.num_nodes; node += gridDim.x) {
-    const int64_t r0 = g.row_ptr[4 * node];
-    const int deg4 = (int)(g.row_ptr[4 * node + 1] - r0);  // 4 * deg(node)
-    const int blk = 4 * deg4;
-    const int64_t k0 = g.inc_ptr[node], k1 = g.inc_ptr[node + 1];
+  __shared__ int64_t srow[4 * NC + 1];
+  __shared__ int64_t sinc[NC + 1];
+  __shared__ double racc[4 * NC];
+  __shared__ uint32_t spos2[RC * NN / 2];  // posi rows of the chunk, two uint16 per word
+  __shared__ uint8_t rnode[RC];            // run-local node of each record of the chunk
+  const uint16_t* spos = reinterpret_cast<const uint16_t*>(spos2);
+  const int tid = threadIdx.x;
+  const int eq = tid >> 6, lane = tid & 63;  // this wave's row of every node
+  for (int64_t w = blockIdx.x; w < g.num_ranges; w += gridDim.x) {
+    const int64_t a0 = g.ranges[w];
+    const int na = (int)(g.ranges[w + 1] - a0);
+    for (int t = tid; t <= 4 * na; t += NT) srow[t] = g.row_ptr[4 * a0 + t];
+    for (int t = tid; t <= na; t += NT) sinc[t] = g.inc_ptr[a0 + t];
+    for (int t = tid; t < 4 * na; t += NT) racc[t] = 0.0;
+    __syncthreads();
+    const int64_t r0 = srow[0];
+    const int nv = (int)(srow[4 * na] - r0);
+    const int64_t k0 = sinc[0], k1 = sinc[na];
     if (g.vals)
-      for (int t = lane; t < blk; t += 64) acc[t] = 0.0;
-    if (lane < 4) racc[lane] = 0.0;
-    for (int64_t kb = k0; kb < k1; kb += 64) {
-      const int m = (int)min((int64_t)64, k1 - kb);
-      const double* em = g.vals ? g.em + kb * RB : nullptr;
-      const int total = g.vals ? m * RB : 0;
-      double src[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int v = u * 64 + lane;
-        src[u] = v < total ? em[v] : 0.0;
+      for (int t = tid; t < nv; t += NT) acc[t] = 0.0;
+    for (int64_t kb = k0; kb < k1; kb += RC) {
+      const int m = (int)min((int64_t)RC, k1 - kb);
+      // this wave's values: row `eq` of each record, ND per record
+      const TE* em = nullptr;
+      if (g.vals) {
+        if constexpr (std::is_same<TE, float>::value) em = g.em32 + kb * RB + eq * ND;
+        else em = g.em + kb * RB + eq * ND;
       }
-      if (g.vals)
-        for (int t = lane; t < m * NN; t += 64) spos[t / NN][t % NN] = g.posi[kb * NN + t];
-      __syncthreads();  // acc / racc cleared, spos staged
-      for (int v = lane; v < 4 * m; v += 64) atomicAdd(&racc[v & 3], g.emr[kb * 4 + v]);
-      for (int v0 = 0; v0 < total; v0 += U * 64) {
-        double nxt[U];
+      const int total = g.vals ? m * ND : 0;
+      // loads are unconditional (index clamped into the chunk) so all U issue
+      // back to back; out-of-range lanes are dropped at the add
+      TE src[U];
+      if (total > 0)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int v = v0 + (U + u) * 64 + lane;
-          nxt[u] = v < total ? em[v] : 0.0;
+          const int q = min(u * 64 + lane, total - 1);
+          src[u] = em[(q / ND) * RB + q % ND];
         }
+      if (g.vals) {
+        // kb * NN and m * NN are even (NN is 4 or 10): whole 32-bit words
+        const uint32_t* pw = reinterpret_cast<const uint32_t*>(g.posi + kb * NN);
+        const int nw = m * NN / 2;
+        for (int t0 = 0; t0 < nw; t0 += 4 * NT) {
+          uint32_t v4[4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int v = v0 + u * 64 + lane;
-          if (v < total) {
-            const int j = v / RB, t = v % RB;
-            const int i = t / ND, c = t % ND;
-            atomicAdd(&acc[i * deg4 + 4 * (int)spos[j][c >> 2] + (c & 3)], src[u]);
+          for (int u = 0; u < 4; ++u) v4[u] = pw[min(t0 + u * NT + tid, nw - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * NT + tid;
+            if (t < nw) spos2[t] = v4[u];
           }
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) src[u] = nxt[u];
       }
-      __syncthreads();  // spos is rewritten by the next chunk; acc complete
+      for (int t = tid; t < na; t += NT) {
+        const int64_t lo = max(sinc[t], kb), hi = min(sinc[t + 1], kb + m);
+        for (int64_t k = lo; k < hi; ++k) rnode[k - kb] = (uint8_t)t;
+      }
+      double rv[RC / 64];
+      if (g.R)
+#pragma unroll
+        for (int u = 0; u < RC / 64; ++u) rv[u] = g.emr[(kb + min(u * 64 + lane, m - 1)) * 4 + eq];
+      __syncthreads();  // acc / racc cleared, spos / rnode staged
+#pragma unroll
+      for (int u = 0; u < RC / 64; ++u) {
+        const int j = u * 64 + lane;
+        if (g.R && j < m) atomicAdd(&racc[4 * rnode[j] + eq], rv[u]);
+      }
+      // double-buffered stream: batch A is summed while batch B is in flight
+      // and the other way round (no register copies of loads in flight)
+      auto add_batch = [&](const TE* buf, int qb) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int q = qb + u * 64 + lane;
+          if (q < total) {
+            const int j = q / ND, c = q % ND;
+            const int slot = (int)(srow[4 * rnode[j] + eq] - r0) + 4 * (int)spos[j * NN + (c >> 2)] + (c & 3);
+            atomicAdd(&acc[slot], (double)buf[u]);
+          }
+        }
+      };
+      auto load_batch = [&](TE* buf, int qb) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int q = min(qb + u * 64 + lane, total - 1);
+          buf[u] = em[(q / ND) * RB + q % ND];
+        }
+      };
+      for (int q0 = 0; q0 < total; q0 += 2 * U * 64) {
+        TE nxt[U];
+        load_batch(nxt, q0 + U * 64);
+        add_batch(src, q0);
+        if (q0 + U * 64 >= total) break;
+        load_batch(src, q0 + 2 * U * 64);
+        add_batch(nxt, q0 + U * 64);
+      }
+      __syncthreads();  // spos / rnode are rewritten by the next chunk; acc complete
     }
-    if (g.R && lane < 4) {
-      const int64_t row = 4 * node + lane;
-      g.R[row] = (g.overwrite_R ? 0.0 : g.R[row]) + racc[lane];
-    }
+    if (g.R)
+      for (int t = tid; t < 4 * na; t += NT) {
+        const int64_t row = 4 * a0 + t;
+        g.R[row] = (g.overwrite_R ? 0.0 : g.R[row]) + racc[t];
+      }
     if (g.vals) {
       double* out = g.vals + r0;
       if (g.overwrite_vals)
-        for (int t = lane; t < blk; t += 64) out[t] = acc[t];
+        for (int t = tid; t < nv; t += NT) out[t] = acc[t];
       else
-        for (int t = lane; t < blk; t += 64) out[t] += acc[t];
+        for (int t = tid; t < nv; t += NT) out[t] += acc[t];
     }
     __syncthreads();
   }
 }
 
-template <int NN, int NQ, int MODEL, int KIND>
+template <int NN, int NQ, int MODEL, int KIND, typename TE>
 int occupancy() {
   static const int occ = [] {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&element_kernel<NN, NQ, MODEL, KIND>), Layout<NN, NQ, MODEL>::T, 0) !=
-            hipSuccess ||
+            &n, reinterpret_cast<const void*>(&element_kernel<NN, NQ, MODEL, KIND, TE>), Layout<NN, NQ, MODEL>::T,
+            0) != hipSuccess ||
         n < 1)
       n = 1;
     return n;
@@ -535,30 +607,37 @@ int occupancy() {
 
 // Persistent grid: the workgroups that fit on the device at once (a multiple
 // of 8 so the XCD slab schedule applies), at most one per element group.
-template <int NN, int NQ, int MODEL, int KIND>
+template <int NN, int NQ, int MODEL, int KIND, typename TE>
 int resident_grid(int64_t ne) {
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const int64_t groups = (ne + Layout<NN, NQ, MODEL>::G - 1) / Layout<NN, NQ, MODEL>::G;
-  int64_t want = std::min<int64_t>((int64_t)cus * occupancy<NN, NQ, MODEL, KIND>(), groups);
+  int64_t want = std::min<int64_t>((int64_t)cus * occupancy<NN, NQ, MODEL, KIND, TE>(), groups);
   return (int)std::max<int64_t>(8, (want + 7) / 8 * 8);
 }
 
-template <int NN, int NQ, int MODEL, int KIND>
+template <int NN, int NQ, int MODEL, int KIND, typename TE>
 hipError_t launch4(const AsmArgs& a, int blocks, hipStream_t s) {
-  if (blocks <= 0) blocks = resident_grid<NN, NQ, MODEL, KIND>(a.num_elems);
-  hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, KIND>), dim3(blocks), dim3(Layout<NN, NQ, MODEL>::T), 0, s, a);
+  if (blocks <= 0) blocks = resident_grid<NN, NQ, MODEL, KIND, TE>(a.num_elems);
+  hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, KIND, TE>), dim3(blocks), dim3(Layout<NN, NQ, MODEL>::T), 0, s,
+                     a);
   return hipGetLastError();
 }
 
+// a.em32 set: FP32 element matrices (Jacobian kinds only).
 template <int NN, int NQ, int MODEL>
 hipError_t launch3(int kind, const AsmArgs& a, int blocks, hipStream_t s) {
+  const bool f32 = a.em32 != nullptr;
   switch (kind) {
-    case ASM_RESIDUAL: return launch4<NN, NQ, MODEL, ASM_RESIDUAL>(a, blocks, s);
-    case ASM_JACOBIAN: return launch4<NN, NQ, MODEL, ASM_JACOBIAN>(a, blocks, s);
-    case ASM_ADJOINT: return launch4<NN, NQ, MODEL, ASM_ADJOINT>(a, blocks, s);
-    case ASM_LOCALIZE: return launch4<NN, NQ, MODEL, ASM_LOCALIZE>(a, blocks, s);
+    case ASM_RESIDUAL: return launch4<NN, NQ, MODEL, ASM_RESIDUAL, double>(a, blocks, s);
+    case ASM_JACOBIAN:
+      return f32 ? launch4<NN, NQ, MODEL, ASM_JACOBIAN, float>(a, blocks, s)
+                 : launch4<NN, NQ, MODEL, ASM_JACOBIAN, double>(a, blocks, s);
+    case ASM_ADJOINT:
+      return f32 ? launch4<NN, NQ, MODEL, ASM_ADJOINT, float>(a, blocks, s)
+                 : launch4<NN, NQ, MODEL, ASM_ADJOINT, double>(a, blocks, s);
+    case ASM_LOCALIZE: return launch4<NN, NQ, MODEL, ASM_LOCALIZE, double>(a, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -700,14 +779,36 @@ hipError_t launch_element(int kind, int order, int quad_order, const AsmArgs& a,
 }
 
 hipError_t launch_gather(int order, const GatherArgs& g, int blocks, hipStream_t s) {
-  if (g.num_nodes == 0) return hipSuccess;
-  if (blocks < 1) return hipErrorInvalidValue;
-  const size_t lds = g.vals ? sizeof(double) * 16 * (size_t)g.max_degree : 0;
-  if (order == 1)
-    hipLaunchKernelGGL(gather_kernel<4>, dim3(blocks), dim3(64), lds, s, g);
+  if (g.num_ranges == 0) return hipSuccess;
+  if (blocks < 1 || g.cap < 1) return hipErrorInvalidValue;
+  const size_t lds = g.vals ? sizeof(double) * (size_t)g.cap : 0;
+  const bool f32 = g.em32 != nullptr;
+  if (order == 1 && f32)
+    hipLaunchKernelGGL((gather_kernel<4, float>), dim3(blocks), dim3(kGatherThreads), lds, s, g);
+  else if (order == 1)
+    hipLaunchKernelGGL((gather_kernel<4, double>), dim3(blocks), dim3(kGatherThreads), lds, s, g);
+  else if (f32)
+    hipLaunchKernelGGL((gather_kernel<10, float>), dim3(blocks), dim3(kGatherThreads), lds, s, g);
   else
-    hipLaunchKernelGGL(gather_kernel<10>, dim3(blocks), dim3(64), lds, s, g);
+    hipLaunchKernelGGL((gather_kernel<10, double>), dim3(blocks), dim3(kGatherThreads), lds, s, g);
   return hipGetLastError();
+}
+
+int gather_nodes_per_range() { return kGatherNodes; }
+
+int gather_grid(int order, bool f32, int cap, int64_t num_ranges) {
+  int dev = 0, cus = 256, occ = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const void* f = order == 1 ? (f32 ? reinterpret_cast<const void*>(&gather_kernel<4, float>)
+                                    : reinterpret_cast<const void*>(&gather_kernel<4, double>))
+                             : (f32 ? reinterpret_cast<const void*>(&gather_kernel<10, float>)
+                                    : reinterpret_cast<const void*>(&gather_kernel<10, double>));
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, kGatherThreads, sizeof(double) * (size_t)cap) !=
+          hipSuccess ||
+      occ < 1)
+    occ = 1;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(num_ranges, (int64_t)cus * occ));
 }
 
 hipError_t launch_qoi(int order, int quad_order, const QoiArgs& a, hipStream_t s) {

@@ -159,6 +159,10 @@ struct goal_disc_s {
   std::vector<int64_t> node_adj_ptr;
   int max_degree = 0;
   int64_t num_dofs = 0, nnz = 0;
+  // gather plan: runs of consecutive nodes whose CSR values fit `gather_cap`
+  int gather_cap = 0;
+  int64_t num_ranges = 0;
+  DevBuf<int32_t> ranges;
   DevBuf<double> coords, x;
   DevBuf<int32_t> conn, elem_set, cols, vertex_nodes, inc, rec;
   DevBuf<uint16_t> pos, posi;
@@ -187,6 +191,7 @@ struct goal_mech_s {
   goal_states st = nullptr;
   int model = 0, stab = 1, quad_order = 1, num_sets = 1;
   int blocks = 0, xcd_remap = 1, check = 1;
+  int tangent = 64;  // 32: FP32 element matrices for the Jacobian kinds
   DevBuf<goal::MatParams> mats;
   DevBuf<int32_t> status;
   DevBuf<double> elem_value;
@@ -224,6 +229,24 @@ void disc_build(goal_disc d) {
   d->inc_ptr.upload(g.inc_ptr.data(), (int64_t)g.inc_ptr.size());
   d->inc.upload(g.inc.data(), (int64_t)g.inc.size());
   d->rec.upload(g.rec.data(), (int64_t)g.rec.size());
+  {
+    // 16 * deg(a) CSR values per node; at least 2048 per run, and always one node
+    d->gather_cap = std::max(2048, 16 * g.max_degree);
+    const int nmax = goal::gather_nodes_per_range();
+    std::vector<int32_t> r{0};
+    int64_t cur = 0;
+    for (int64_t a = 0; a < m.num_nodes; ++a) {
+      const int64_t v = 16 * (g.node_adj_ptr[a + 1] - g.node_adj_ptr[a]);
+      if (a > r.back() && (cur + v > d->gather_cap || a - r.back() >= nmax)) {
+        r.push_back((int32_t)a);
+        cur = 0;
+      }
+      cur += v;
+    }
+    r.push_back((int32_t)m.num_nodes);
+    d->num_ranges = (int64_t)r.size() - 1;
+    d->ranges.upload(r.data(), (int64_t)r.size());
+  }
   d->posi.upload(g.posi.data(), (int64_t)g.posi.size());
   DevBuf<int64_t> adj_ptr;
   DevBuf<int32_t> adj;
@@ -294,11 +317,6 @@ goal::AsmArgs asm_args(goal_mech m) {
 // 0 lets the launcher size a persistent grid from the kernel's occupancy.
 int asm_blocks(goal_mech m) { return m->blocks; }
 
-int gather_blocks(goal_disc d) {
-  int cus = 256;
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx().device);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(d->mesh.num_nodes, (int64_t)cus * 32));
-}
 
 // Fill whatever zero_* left pending (before any access other than assembly).
 void materialize(goal_solinfo s) {
@@ -316,7 +334,9 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
   if (want_vals && m->em.n < d->mesh.num_elems * nd * nd) m->em.alloc(d->mesh.num_elems * nd * nd);
   a.rec = d->rec.p;
   a.emr = m->emr.p;
-  a.em = want_vals ? m->em.p : nullptr;
+  const bool f32 = want_vals && m->tangent == 32;
+  a.em = (want_vals && !f32) ? m->em.p : nullptr;
+  a.em32 = f32 ? reinterpret_cast<float*>(m->em.p) : nullptr;
   if (m->timing) HIPCHK(hipEventRecord(m->ev[0], stream()));
   HIPCHK(goal::launch_element(kind, d->mesh.order, m->quad_order, a, asm_blocks(m), stream()));
   if (m->timing) HIPCHK(hipEventRecord(m->ev[1], stream()));
@@ -329,18 +349,22 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
     s->zero_vals = false;
   }
   goal::GatherArgs g{};
-  g.num_nodes = d->mesh.num_nodes;
+  g.num_ranges = d->num_ranges;
+  g.ranges = d->ranges.p;
+  g.cap = d->gather_cap;
   g.inc_ptr = d->inc_ptr.p;
   g.posi = d->posi.p;
   g.row_ptr = d->row_ptr.p;
   g.emr = m->emr.p;
-  g.em = want_vals ? m->em.p : nullptr;
+  g.em = a.em;
+  g.em32 = a.em32;
   g.R = s->R.p;
   g.vals = want_vals ? s->vals.p : nullptr;
-  g.max_degree = d->max_degree;
   g.overwrite_R = s->zero_R ? 1 : 0;
   g.overwrite_vals = (want_vals && s->zero_vals) ? 1 : 0;
-  HIPCHK(goal::launch_gather(d->mesh.order, g, gather_blocks(d), stream()));
+  HIPCHK(goal::launch_gather(d->mesh.order, g,
+                             goal::gather_grid(d->mesh.order, g.em32 != nullptr, d->gather_cap, d->num_ranges),
+                             stream()));
   if (m->timing) {
     HIPCHK(hipEventRecord(m->ev[2], stream()));
     m->timed = true;
@@ -943,6 +967,9 @@ int goal_mech_set_option(goal_mech m, const char* key, int64_t value) {
       m->check = value ? 1 : 0;
     } else if (k == "stabilization") {
       m->stab = value ? 1 : 0;
+    } else if (k == "tangent") {
+      require(value == 32 || value == 64, "tangent must be 32 or 64");
+      m->tangent = (int)value;
     } else if (k == "timing") {
       if (value)
         for (auto& e : m->ev)

@@ -499,34 +499,36 @@ __device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st,
 //   Pressure DOF of node b (dp = N_b, d grad p = G), Mixed and PResidual:
 //     dP_iJ = N_b J Finv_Ji,   dr1 = N_b / kappa,   ds3_j = tau J (Cinv G)_j
 // These are the columns of the point tangent summed against G; scaling G and
-// N_b by w dv scales the result.
-__device__ inline void nh_direction(const NHBase& b, int k, double N, const double* G, double* C) {
+// N_b by w dv scales the result.  T = float evaluates the FP32 tangent from the
+// FP64 point state.
+template <typename T>
+__device__ inline void nh_direction(const NHBase& b, int k, T N, const T* G, T* C) {
+  const T c = (T)b.c, J = (T)b.J, tJ = (T)(b.tau * b.J);
   if (k < 3) {
-    double FG[3], PG[3], CG[3];
+    T FG[3], PG[3], CG[3];
     for (int i = 0; i < 3; ++i) {
-      FG[i] = b.F[3 * i] * G[0] + b.F[3 * i + 1] * G[1] + b.F[3 * i + 2] * G[2];
-      PG[i] = b.P[3 * i] * G[0] + b.P[3 * i + 1] * G[1] + b.P[3 * i + 2] * G[2];
-      CG[i] = b.Cinv[3 * i] * G[0] + b.Cinv[3 * i + 1] * G[1] + b.Cinv[3 * i + 2] * G[2];
+      FG[i] = (T)b.F[3 * i] * G[0] + (T)b.F[3 * i + 1] * G[1] + (T)b.F[3 * i + 2] * G[2];
+      PG[i] = (T)b.P[3 * i] * G[0] + (T)b.P[3 * i + 1] * G[1] + (T)b.P[3 * i + 2] * G[2];
+      CG[i] = (T)b.Cinv[3 * i] * G[0] + (T)b.Cinv[3 * i + 1] * G[1] + (T)b.Cinv[3 * i + 2] * G[2];
     }
-    const double gk = b.Fi[k] * G[0] + b.Fi[3 + k] * G[1] + b.Fi[6 + k] * G[2];
-    const double u = (2.0 / 3.0) * b.c * FG[k];
+    const T gk = (T)b.Fi[k] * G[0] + (T)b.Fi[3 + k] * G[1] + (T)b.Fi[6 + k] * G[2];
+    const T u = (T)(2.0 / 3.0) * c * FG[k];
     for (int i = 0; i < 3; ++i) {
-      const double t = b.c * FG[i] - PG[i];
-      for (int J = 0; J < 3; ++J)
-        C[3 * i + J] = gk * b.Q[3 * i + J] + b.Fi[3 * J + k] * t - u * b.Fi[3 * J + i] + (i == k ? b.c * G[J] : 0.0);
+      const T t = c * FG[i] - PG[i];
+      for (int jj = 0; jj < 3; ++jj)
+        C[3 * i + jj] = gk * (T)b.Q[3 * i + jj] + (T)b.Fi[3 * jj + k] * t - u * (T)b.Fi[3 * jj + i] +
+                        (i == k ? c * G[jj] : (T)0);
     }
-    C[9] = b.beta * gk;
-    const double vG = b.v[0] * G[0] + b.v[1] * G[1] + b.v[2] * G[2];
-    const double tJ = b.tau * b.J;
-    for (int j = 0; j < 3; ++j) C[10 + j] = tJ * (gk * b.v[j] - b.h[k] * CG[j] - b.Fi[3 * j + k] * vG);
+    C[9] = (T)b.beta * gk;
+    const T vG = (T)b.v[0] * G[0] + (T)b.v[1] * G[1] + (T)b.v[2] * G[2];
+    for (int j = 0; j < 3; ++j) C[10 + j] = tJ * (gk * (T)b.v[j] - (T)b.h[k] * CG[j] - (T)b.Fi[3 * j + k] * vG);
   } else {
-    const double NJ = N * b.J;
+    const T NJ = N * J;
     for (int i = 0; i < 3; ++i)
-      for (int J = 0; J < 3; ++J) C[3 * i + J] = NJ * b.Fi[3 * J + i];
-    C[9] = N * b.ikappa;
-    const double tJ = b.tau * b.J;
+      for (int jj = 0; jj < 3; ++jj) C[3 * i + jj] = NJ * (T)b.Fi[3 * jj + i];
+    C[9] = N * (T)b.ikappa;
     for (int j = 0; j < 3; ++j)
-      C[10 + j] = tJ * (b.Cinv[3 * j] * G[0] + b.Cinv[3 * j + 1] * G[1] + b.Cinv[3 * j + 2] * G[2]);
+      C[10 + j] = tJ * ((T)b.Cinv[3 * j] * G[0] + (T)b.Cinv[3 * j + 1] * G[1] + (T)b.Cinv[3 * j + 2] * G[2]);
   }
 }
 

@@ -40,6 +40,7 @@ struct AsmArgs {
   // ASM_ADJOINT the row blocks of the transposed element matrix).
   double* emr;
   double* em;               // null: residual only
+  float* em32;              // FP32 element matrices instead of em (FP32 tangent)
   // per quadrature point history, structure of arrays: comp * (ne*nq) + e*nq + q
   const double* Fp_old;
   const double* eqps_old;
@@ -58,15 +59,17 @@ struct AsmArgs {
 
 // Node-owner gather of element residuals / matrices into R and the CSR values.
 struct GatherArgs {
-  int64_t num_nodes;
+  int64_t num_ranges;
+  const int32_t* ranges;    // [num_ranges + 1] first node of each run of nodes
+  int cap;                  // LDS budget: max CSR values of one run (>= 16 * max degree)
   const int64_t* inc_ptr;   // [num_nodes + 1] node -> its incidence records
   const uint16_t* posi;     // [records][npe] position of each element node in adj(node)
   const int64_t* row_ptr;   // [num_dofs + 1]
   const double* emr;        // node-major, see AsmArgs
   const double* em;         // null: residual only
+  const float* em32;        // FP32 element matrices instead of em
   double* R;                // null: skip
   double* vals;             // null: skip
-  int max_degree;
   int overwrite_R;          // 1: R = sum (R was pending zero), 0: R += sum
   int overwrite_vals;
 };
@@ -94,6 +97,10 @@ struct QoiArgs {
 // Launchers (goal_assembly.hip).  order: 1 or 2; quad_order: 1 or 2.
 hipError_t launch_element(int kind, int order, int quad_order, const AsmArgs& a, int blocks, hipStream_t s);
 hipError_t launch_gather(int order, const GatherArgs& g, int blocks, hipStream_t s);
+// Nodes per run the gather kernel accepts (plan_gather_ranges must respect it).
+int gather_nodes_per_range();
+// Persistent grid for the gather: the workgroups resident at once, at most one per run.
+int gather_grid(int order, bool f32, int cap, int64_t num_ranges);
 hipError_t launch_qoi(int order, int quad_order, const QoiArgs& a, hipStream_t s);
 
 // Utility kernels (goal_util.hip)

@@ -165,6 +165,10 @@ int goal_mech_destroy(goal_mech m);
 /* Options: "blocks" (workgroups per launch, 0 = auto), "xcd_remap" (0/1),
  * "check" (1: synchronise after each assembly and report J2 failures),
  * "stabilization" (0/1, overrides the value given at creation),
+ * "tangent" (64: FP64 element matrices, the default; 32: element matrices
+ * formed and stored in FP32 and summed into the FP64 CSR -- the residual stays
+ * FP64, so Newton still converges to the FP64 tolerance, in more iterations
+ * at worst),
  * "timing" (1: record device events around the element and gather kernels of
  * each assembly, read back with goal_mech_timing). */
 int goal_mech_set_option(goal_mech m, const char* key, int64_t value);

@@ -259,6 +259,7 @@ struct MechanicsParams {
   bool stabilization = true;          // src/goal_mechanics.cpp:33
   std::vector<goal_material> materials;  // one per element set
   int quad_order = 1;                 // 1: the reference's one point; 2: four points
+  int tangent = 64;                   // 32: FP32 element matrices (FP64 residual)
 };
 
 inline int model_id(std::string const& m) {
@@ -280,6 +281,7 @@ class Mechanics {
     check(goal_states_create(d->handle(), model, p.quad_order, &states_));
     check(goal_mech_create(d->handle(), states_, (int)mats.size(), mats.data(), p.stabilization ? 1 : 0,
                            p.quad_order, &mech_));
+    check(goal_mech_set_option(mech_, "tangent", p.tangent));
   }
   ~Mechanics() {
     if (mech_) goal_mech_destroy(mech_);

@@ -4,7 +4,7 @@
 // error estimate.  The linear solves use a dense LU, adequate for the small
 // meshes this program is run on.  Prints one JSON object.
 //
-// usage: surface_test <order> <quad_order> <model> <n> <steps>
+// usage: surface_test <order> <quad_order> <model> <n> <steps> [tangent bits: 64 | 32]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -62,11 +62,13 @@ int main(int argc, char** argv) {
   }
   const int order = std::atoi(argv[1]), quad = std::atoi(argv[2]), model = std::atoi(argv[3]);
   const int n = std::atoi(argv[4]), steps = std::atoi(argv[5]);
+  const int tangent = argc > 6 ? std::atoi(argv[6]) : 64;
   try {
     auto disc = goal::Disc::box(n, n, n, 1.0, 1.0, 1.0, order, 0.1, 11);
     goal::MechanicsParams mp;
     mp.model = model == 2 ? "J2" : model == 0 ? "elastic" : "neohookean";
     mp.quad_order = quad;
+    mp.tangent = tangent;
     goal_material mat{};
     if (model == 2) mat = goal_material{GOAL_MODEL_J2, 200.0, 0.3, 2.0, 0.25, 1.0};
     else mat = goal_material{GOAL_MODEL_NEOHOOKEAN, 1000.0, 0.25, 0.0, 0.0, 1.0};

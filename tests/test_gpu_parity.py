@@ -96,6 +96,27 @@ def test_jacobian_adjoint(order, quad, model):
     assert rel(si.vals(), Vo) < RTOL
 
 
+# FP32 tangent: element matrices formed and stored in FP32, summed into the
+# FP64 CSR.  Bar for dRdu: 2e-6 normwise relative (FP32 rounding, ~1e-7, over
+# a few tens of accumulated terms); R and the states stay FP64 (RTOL).
+F32_TOL = 2e-6
+
+
+@pytest.mark.parametrize("order,quad,model", CASES)
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_jacobian_fp32_tangent(order, quad, model, adjoint):
+    disc, st, mech, x, om, ost, mats = make(order, quad, model)
+    mech.set_option("tangent", 32)
+    si = g.SolInfo(disc)
+    si.zero_all()
+    mech.jacobian(si, adjoint=adjoint, save_states=not adjoint)
+    kind = O.K_ADJOINT if adjoint else O.K_JACOBIAN
+    Ro, Vo = O.assemble(kind, om, model, mats, x, ost, quad_order=quad, save=not adjoint)
+    assert rel(si.R(), Ro) < RTOL
+    e = rel(si.vals(), Vo)
+    assert 0.0 < e < F32_TOL  # rounded to FP32, and no further
+
+
 @pytest.mark.parametrize("order,quad,model", [(1, 1, 1), (2, 2, 1), (1, 1, 2), (2, 2, 2)])
 def test_qoi(order, quad, model):
     disc, st, mech, x, om, ost, mats = make(order, quad, model)
