@@ -95,14 +95,20 @@ def _find(dirpath, suffix):
     raise FileNotFoundError(f"no *{suffix} under {dirpath}")
 
 
-def _short(name: str) -> str:
+def _short(name: str, tangent: int = 64):
+    """Short name of a kernel, or None for an assembly kernel instantiated for
+    the other element-matrix precision."""
     for k in ASSEMBLY_KERNELS + ("copy_kernel", "fill_kernel", "expand_csr_kernel"):
         if k in name:
+            if k in ASSEMBLY_KERNELS:
+                want, other = ("double", "float") if tangent == 64 else ("float", "double")
+                if other in name and want not in name:
+                    return None
             return k
     return name
 
 
-def summarize(kt_dir, fetch_dir, write_dir, tag, commands):
+def summarize(kt_dir, fetch_dir, write_dir, tag, commands, tangent=64):
     """Turn rocprofv3 CSV output (kernel trace + stats, FETCH_SIZE pass,
     WRITE_SIZE pass) into the text summaries kept under profiles/."""
     stats = _read_csv(_find(kt_dir, "kernel_stats.csv"))
@@ -111,7 +117,9 @@ def summarize(kt_dir, fetch_dir, write_dir, tag, commands):
         for r in _read_csv(_find(d, "counter_collection.csv")):
             if r["Counter_Name"] != which:
                 continue
-            k = _short(r["Kernel_Name"])
+            k = _short(r["Kernel_Name"], tangent)
+            if k is None:
+                continue
             per.setdefault(k, {}).setdefault(which, []).append(float(r["Counter_Value"]))
     mean = {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in per.items()}
     # calibration: the copy kernel reads and writes 8 * NCOPY bytes per launch
@@ -120,7 +128,7 @@ def summarize(kt_dir, fetch_dir, write_dir, tag, commands):
     cal_write = cbytes / (mean["copy_kernel"]["WRITE_SIZE"] * 1024.0)
     kern = {}
     for r in stats:
-        k = _short(r["Name"])
+        k = _short(r["Name"], tangent)
         if k in ASSEMBLY_KERNELS or k == "copy_kernel":
             kern[k] = {"calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"])}
     out = {"tag": tag, "commands": commands, "calibration": {"fetch": cal_fetch, "write": cal_write},
@@ -139,7 +147,10 @@ def summarize(kt_dir, fetch_dir, write_dir, tag, commands):
     for r in stats:
         lines.append(f"| `{r['Name'][:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
                      f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} |")
-    lines += ["", "## FETCH_SIZE / WRITE_SIZE (KiB per dispatch, mean)", "",
+    inst = "double" if tangent == 64 else "float"
+    lines += ["", f"The benchmark also times the other element-matrix precision in the same run; the traffic "
+              f"and the per-kernel times below count only the `{inst}` instantiations of the assembly kernels.",
+              "", "## FETCH_SIZE / WRITE_SIZE (KiB per dispatch, mean)", "",
               f"Calibration: the copy kernel moves {cbytes / 2**30:.0f} GiB each way per launch; the counters read "
               f"{mean['copy_kernel']['FETCH_SIZE'] / 2**20:.3f} GiB fetched and "
               f"{mean['copy_kernel']['WRITE_SIZE'] / 2**20:.3f} GiB written, so FETCH_SIZE is scaled by "
@@ -198,7 +209,7 @@ def main() -> int:
                 f"{base} --steps 2 --warmup 1 --no-cpu-baseline",
                 f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {args.summarize[2]} -o run -- "
                 f"{base} --steps 2 --warmup 1 --no-cpu-baseline"]
-        return summarize(*args.summarize, profile_tag(args.n, args.order, args.tangent), cmds)
+        return summarize(*args.summarize, profile_tag(args.n, args.order, args.tangent), cmds, args.tangent)
 
     if args.gpus != 1:
         print(
