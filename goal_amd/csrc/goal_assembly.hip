@@ -438,9 +438,26 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     if constexpr (std::is_same<TE, float>::value) em = a.em32;
     else em = a.em;
     if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
-      for (int t = tid; t < nv * ND; t += T) {
-        const int el = t / ND, col = t % ND;
-        emit_column_nh<NN, NQ, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], col, s.rec[el], em);
+      if constexpr (T == 256 && G * NN <= 64 && std::is_same<TE, double>::value) {
+        // displacement columns on waves 0-2, pressure columns on wave 3: the
+        // closed-form column takes a different (shorter) path for pressure,
+        // and keeping the two apart leaves no wave running both (measured:
+        // faster for FP64 columns, slower for FP32 ones)
+        const int w = tid >> 6, l = tid & 63;
+        if (w < 3) {
+          for (int t = tid; t < 3 * nv * NN; t += 192) {
+            const int el = t / (3 * NN), r = t % (3 * NN);
+            emit_column_nh<NN, NQ, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], 4 * (r / 3) + r % 3, s.rec[el], em);
+          }
+        } else if (l < nv * NN) {
+          const int el = l / NN, b = l % NN;
+          emit_column_nh<NN, NQ, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], 4 * b + 3, s.rec[el], em);
+        }
+      } else {
+        for (int t = tid; t < nv * ND; t += T) {
+          const int el = t / ND, col = t % ND;
+          emit_column_nh<NN, NQ, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], col, s.rec[el], em);
+        }
       }
     } else if constexpr (JAC) {
       for (int t = tid; t < nv * ND; t += T) {

@@ -511,13 +511,16 @@ __device__ inline void nh_direction(const NHBase& b, int k, T N, const T* G, T* 
       PG[i] = (T)b.P[3 * i] * G[0] + (T)b.P[3 * i + 1] * G[1] + (T)b.P[3 * i + 2] * G[2];
       CG[i] = (T)b.Cinv[3 * i] * G[0] + (T)b.Cinv[3 * i + 1] * G[1] + (T)b.Cinv[3 * i + 2] * G[2];
     }
+    // k-indexed quantities come from LDS (computed address), not from
+    // register arrays, so no per-lane selects
     const T gk = (T)b.Fi[k] * G[0] + (T)b.Fi[3 + k] * G[1] + (T)b.Fi[6 + k] * G[2];
-    const T u = (T)(2.0 / 3.0) * c * FG[k];
+    const T FGk = (T)b.F[3 * k] * G[0] + (T)b.F[3 * k + 1] * G[1] + (T)b.F[3 * k + 2] * G[2];
+    const T u = (T)(2.0 / 3.0) * c * FGk;
     for (int i = 0; i < 3; ++i) {
       const T t = c * FG[i] - PG[i];
+      const T dik = (i == k) ? c : (T)0;
       for (int jj = 0; jj < 3; ++jj)
-        C[3 * i + jj] = gk * (T)b.Q[3 * i + jj] + (T)b.Fi[3 * jj + k] * t - u * (T)b.Fi[3 * jj + i] +
-                        (i == k ? c * G[jj] : (T)0);
+        C[3 * i + jj] = gk * (T)b.Q[3 * i + jj] + (T)b.Fi[3 * jj + k] * t - u * (T)b.Fi[3 * jj + i] + dik * G[jj];
     }
     C[9] = (T)b.beta * gk;
     const T vG = (T)b.v[0] * G[0] + (T)b.v[1] * G[1] + (T)b.v[2] * G[2];
