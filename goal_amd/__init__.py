@@ -45,7 +45,7 @@ class GoalError(RuntimeError):
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -m goal_amd.build` "
+            f"{LIB_PATH} is missing: build it with `python goal_amd/build.py` "
             "(or __graft_entry__.build()); there is no non-HIP fallback"
         )
     return C.CDLL(str(LIB_PATH))
@@ -122,6 +122,8 @@ _sig("goal_solinfo_set", _i32, _p, _p, _p, _p)
 _sig("goal_solinfo_dev", _i32, _p, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p))
 _sig("goal_solinfo_norm_R", _i32, _p, _dp)
 _sig("goal_solinfo_dot_R", _i32, _p, _p, _dp)
+_sig("goal_linear_solve", _i32, _p, _p, _p, C.c_double, _i32, _i32, C.POINTER(_i32), _dp)
+_sig("goal_device_fill", _i32, _p, _i64, C.c_double)
 _sig("goal_mech_create", _i32, _p, _p, _i32, _p, _i32, _i32, C.POINTER(_p))
 _sig("goal_mech_destroy", _i32, _p)
 _sig("goal_mech_set_option", _i32, _p, C.c_char_p, _i64)
@@ -246,6 +248,9 @@ class DeviceArray:
         check(lib.goal_memcpy_h2d(d.ptr, _ptr(a), 8 * a.size))
         return d
 
+    def fill(self, value: float) -> None:
+        check(lib.goal_device_fill(self.ptr, self.n, float(value)))
+
     def numpy(self) -> np.ndarray:
         out = np.empty(self.n, dtype=np.float64)
         check(lib.goal_memcpy_d2h(_ptr(out), self.ptr, 8 * self.n))
@@ -348,6 +353,9 @@ class Disc:
         du = np.ascontiguousarray(du, dtype=np.float64)
         check(lib.goal_disc_add_soln(self.h, _ptr(du)))
 
+    def add_soln_dev(self, du: "DeviceArray") -> None:
+        check(lib.goal_disc_add_soln_dev(self.h, du.ptr))
+
     def soln_dev(self):
         return lib.goal_disc_soln_dev(self.h)
 
@@ -421,6 +429,14 @@ class SolInfo:
         r, m, v = _p(), _p(), _p()
         check(lib.goal_solinfo_dev(self.h, C.byref(r), C.byref(m), C.byref(v)))
         return r.value, m.value, v.value
+
+    def solve(self, b: "DeviceArray", x: "DeviceArray", rtol=1e-10, max_iters=2000, restart=100):
+        """dRdu x = b on the device (GMRES, node block-Jacobi); x holds the
+        initial guess.  Returns (inner iterations, relative residual ||b - Ax|| / ||b - Ax0||)."""
+        it = C.c_int(0)
+        rr = C.c_double(0.0)
+        check(lib.goal_linear_solve(self.h, b.ptr, x.ptr, rtol, max_iters, restart, C.byref(it), C.byref(rr)))
+        return it.value, rr.value
 
     def norm_R(self) -> float:
         out = C.c_double(0.0)

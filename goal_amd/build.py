@@ -1,6 +1,6 @@
 """Build the gfx950 shared library libgoal_amd.so in-tree with hipcc.
 
-Usage: python -m goal_amd.build [-v]
+Usage: python goal_amd/build.py [-v] [output.so]
 """
 from __future__ import annotations
 
@@ -32,7 +32,7 @@ COMMON = [
     f"-I{SRC}",
 ]
 # Device code: CDNA4 only, hardware FP64 atomics, no packed-FP32 instruction
-# selection ("Build").
+# selection (some of its forms have given wrong results on this hardware).
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-munsafe-fp-atomics",
@@ -49,6 +49,7 @@ SOURCES = [
     "goal_assembly.hip",
     "goal_util.hip",
     "goal_dwr.hip",
+    "goal_solve.hip",
 ]
 
 

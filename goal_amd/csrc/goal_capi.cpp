@@ -184,6 +184,9 @@ struct goal_solinfo_s {
   goal_disc d = nullptr;
   DevBuf<double> R, dMdu, vals;
   bool zero_R = false, zero_dMdu = false, zero_vals = false;
+  // linear-solve workspace, kept between solves
+  DevBuf<double> V, t, z, binv, partial, h1, h2, y;
+  DevBuf<int32_t> solve_status;
 };
 
 struct goal_mech_s {
@@ -438,6 +441,13 @@ int goal_device_copy(double* dst, const double* src, int64_t n) {
   return guard([&]() -> int {
     require(dst && src && n >= 0, "goal_device_copy: bad arguments");
     HIPCHK(goal::launch_copy(dst, src, n, stream()));
+    return GOAL_OK;
+  });
+}
+int goal_device_fill(double* p, int64_t n, double value) {
+  return guard([&]() -> int {
+    require(p && n >= 0, "goal_device_fill: bad arguments");
+    HIPCHK(goal::launch_fill(p, n, value, stream()));
     return GOAL_OK;
   });
 }
@@ -880,6 +890,119 @@ int goal_solinfo_dev(goal_solinfo s, double** R, double** dMdu, double** vals) {
     if (R) *R = s->R.p;
     if (dMdu) *dMdu = s->dMdu.p;
     if (vals) *vals = s->vals.p;
+    return GOAL_OK;
+  });
+}
+
+int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, int max_iters, int restart,
+                      int* iters_out, double* relres_out) {
+  return guard([&]() -> int {
+    require(s && b && x && rtol > 0.0 && max_iters >= 0 && restart >= 1 && restart <= 1000,
+            "goal_linear_solve: bad arguments");
+    materialize(s);
+    goal_disc d = s->d;
+    const int64_t n = d->num_dofs, nn = d->mesh.num_nodes;
+    const int m = restart;
+    if (s->V.n < (int64_t)(m + 1) * n) s->V.alloc((int64_t)(m + 1) * n);
+    if (s->t.n < n) s->t.alloc(n);
+    if (s->z.n < n) s->z.alloc(n);
+    if (s->binv.n < 16 * nn) s->binv.alloc(16 * nn);
+    const int nb = goal::multi_dot_blocks(n);
+    if (s->partial.n < (int64_t)(m + 1) * nb) s->partial.alloc((int64_t)(m + 1) * nb);
+    if (s->h1.n < m + 1) s->h1.alloc(m + 1);
+    if (s->h2.n < m + 1) s->h2.alloc(m + 1);
+    if (s->y.n < m + 1) s->y.alloc(m + 1);
+    if (s->solve_status.n < 1) s->solve_status.alloc(1);
+    HIPCHK(hipMemsetAsync(s->solve_status.p, 0, sizeof(int32_t), stream()));
+    const int64_t* rp = d->row_ptr.p;
+    const int32_t* cols = d->cols.p;
+    const double* A = s->vals.p;
+    double* V = s->V.p;
+    double* t = s->t.p;
+    double* z = s->z.p;
+    HIPCHK(goal::launch_block_inverse(nn, rp, cols, A, s->binv.p, s->solve_status.p, stream()));
+    auto norm2 = [&](const double* v) {  // waits
+      HIPCHK(goal::launch_reduce(1, v, nullptr, n, scratch(), result_slot(2), stream()));
+      double r = 0.0;
+      HIPCHK(hipMemcpyAsync(&r, result_slot(2), sizeof(double), hipMemcpyDeviceToHost, stream()));
+      sync();
+      return std::sqrt(r);
+    };
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hh1(m + 1), hh2(m + 1), yv(m + 1);
+    auto Hm = [&](int i, int j) -> double& { return H[(size_t)j * (m + 1) + i]; };
+    int total = 0;
+    double beta0 = -1.0, relres = 0.0;
+    for (;;) {
+      // residual r = b - A x into V_0 (right preconditioning: the Krylov
+      // space is that of A M^{-1}, and the residual GMRES minimizes is b - A x)
+      HIPCHK(goal::launch_spmv(n, rp, cols, A, x, V, stream()));
+      HIPCHK(goal::launch_rsub(n, b, V, stream()));
+      const double beta = norm2(V);
+      if (beta0 < 0.0) beta0 = beta;
+      relres = beta0 > 0.0 ? beta / beta0 : 0.0;
+      if (beta == 0.0 || relres <= rtol || total >= max_iters) break;
+      HIPCHK(goal::launch_scale(n, 1.0 / beta, V, V, stream()));
+      std::fill(g.begin(), g.end(), 0.0);
+      g[0] = beta;
+      int j = 0;
+      bool done = false;
+      while (j < m && total < max_iters) {
+        double* w = V + (int64_t)(j + 1) * n;
+        HIPCHK(goal::launch_block_apply(nn, s->binv.p, V + (int64_t)j * n, t, stream()));
+        HIPCHK(goal::launch_spmv(n, rp, cols, A, t, w, stream()));
+        // classical Gram-Schmidt, twice (DGKS-style reorthogonalization)
+        HIPCHK(goal::launch_multi_dot(n, j + 1, V, n, w, s->partial.p, s->h1.p, stream()));
+        HIPCHK(goal::launch_multi_axpy(n, j + 1, V, n, s->h1.p, -1.0, w, stream()));
+        HIPCHK(goal::launch_multi_dot(n, j + 1, V, n, w, s->partial.p, s->h2.p, stream()));
+        HIPCHK(goal::launch_multi_axpy(n, j + 1, V, n, s->h2.p, -1.0, w, stream()));
+        HIPCHK(hipMemcpyAsync(hh1.data(), s->h1.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, stream()));
+        HIPCHK(hipMemcpyAsync(hh2.data(), s->h2.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, stream()));
+        const double hn = norm2(w);
+        for (int i = 0; i <= j; ++i) Hm(i, j) = hh1[i] + hh2[i];
+        Hm(j + 1, j) = hn;
+        if (hn > 0.0) HIPCHK(goal::launch_scale(n, 1.0 / hn, w, w, stream()));
+        for (int i = 0; i < j; ++i) {
+          const double a = Hm(i, j), c = Hm(i + 1, j);
+          Hm(i, j) = cs[i] * a + sn[i] * c;
+          Hm(i + 1, j) = -sn[i] * a + cs[i] * c;
+        }
+        const double a = Hm(j, j), c = Hm(j + 1, j), r = std::hypot(a, c);
+        cs[j] = r > 0.0 ? a / r : 1.0;
+        sn[j] = r > 0.0 ? c / r : 0.0;
+        Hm(j, j) = r;
+        Hm(j + 1, j) = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        ++j;
+        ++total;
+        relres = std::fabs(g[j]) / beta0;
+        if (relres <= rtol || hn == 0.0) {
+          done = true;
+          break;
+        }
+      }
+      // x += V y with H y = g (upper triangular, j x j)
+      for (int i = j - 1; i >= 0; --i) {
+        double acc = g[i];
+        for (int k = i + 1; k < j; ++k) acc -= Hm(i, k) * yv[k];
+        yv[i] = Hm(i, i) != 0.0 ? acc / Hm(i, i) : 0.0;
+      }
+      if (j > 0) {  // x += M^{-1} (V y)
+        HIPCHK(hipMemcpyAsync(s->y.p, yv.data(), sizeof(double) * j, hipMemcpyHostToDevice, stream()));
+        HIPCHK(goal::launch_fill(z, n, 0.0, stream()));
+        HIPCHK(goal::launch_multi_axpy(n, j, V, n, s->y.p, 1.0, z, stream()));
+        HIPCHK(goal::launch_block_apply(nn, s->binv.p, z, t, stream()));
+        HIPCHK(goal::launch_axpy(x, t, 1.0, n, stream()));
+        sync();  // yv is reused by the next cycle
+      }
+      (void)done;  // the next pass recomputes the true residual and stops
+    }
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, s->solve_status.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    sync();
+    if (iters_out) *iters_out = total;
+    if (relres_out) *relres_out = relres;
+    if (st) return set_error(GOAL_ERR_INVALID, "goal_linear_solve: singular 4x4 diagonal block");
     return GOAL_OK;
   });
 }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 
 namespace goal {
@@ -104,6 +105,21 @@ int gather_grid(int order, bool f32, int cap, int64_t num_ranges);
 hipError_t launch_qoi(int order, int quad_order, const QoiArgs& a, hipStream_t s);
 
 // Utility kernels (goal_util.hip)
+// Device linear solve (goal_solve.hip).
+hipError_t launch_spmv(int64_t nrows, const int64_t* rp, const int32_t* cols, const double* vals, const double* x,
+                       double* y, hipStream_t s);
+hipError_t launch_block_inverse(int64_t nn, const int64_t* rp, const int32_t* cols, const double* vals,
+                                double* binv, int* status, hipStream_t s);
+hipError_t launch_block_apply(int64_t nn, const double* binv, const double* x, double* y, hipStream_t s);
+// out[j] = V_j . w (j < k); partial holds k * multi_dot_blocks(n) doubles
+int multi_dot_blocks(int64_t n);
+hipError_t launch_multi_dot(int64_t n, int k, const double* V, int64_t ldv, const double* w, double* partial,
+                            double* out, hipStream_t s);
+hipError_t launch_multi_axpy(int64_t n, int k, const double* V, int64_t ldv, const double* c, double sign, double* w,
+                             hipStream_t s);
+hipError_t launch_scale(int64_t n, double a, const double* x, double* y, hipStream_t s);
+hipError_t launch_rsub(int64_t n, const double* b, double* y, hipStream_t s);
+
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 hipError_t launch_axpy(double* y, const double* x, double a, int64_t n, hipStream_t s);

@@ -1,0 +1,207 @@
+// Kernels of the device linear solve (GMRES with a node block-Jacobi
+// preconditioner) that stands in for goal's Belos/MueLu solve
+// (src/goal_linear_solve.cpp:22-90).  FP64; every reduction has a fixed
+// order, so a solve is bitwise reproducible.
+
+#include "goal_kernels.hpp"
+
+namespace goal {
+namespace {
+
+constexpr int kSolveThreads = 256;
+
+// y = A x for a CSR matrix, one wavefront per row.
+__global__ __launch_bounds__(kSolveThreads) void spmv_kernel(int64_t nrows, const int64_t* rp, const int32_t* cols,
+                                                             const double* vals, const double* x, double* y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (kSolveThreads / 64) + (threadIdx.x >> 6);
+  if (row >= nrows) return;  // uniform per wave
+  double s = 0.0;
+  for (int64_t k = rp[row] + lane; k < rp[row + 1]; k += 64) s += vals[k] * x[cols[k]];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[row] = s;
+}
+
+// Inverse of each node's 4x4 diagonal block (Gauss-Jordan, partial
+// pivoting).  The block's columns 4a..4a+3 sit contiguously in each of the
+// node's rows; a zero pivot sets status bit 0 and leaves the identity.
+__global__ __launch_bounds__(kSolveThreads) void block_inverse_kernel(int64_t nn, const int64_t* rp,
+                                                                      const int32_t* cols, const double* vals,
+                                                                      double* binv, int* status) {
+  const int64_t a = (int64_t)blockIdx.x * kSolveThreads + threadIdx.x;
+  if (a >= nn) return;
+  double M[4][8];
+  for (int i = 0; i < 4; ++i) {
+    const int64_t row = 4 * a + i;
+    int64_t lo = rp[row], hi = rp[row + 1];
+    const int32_t c0 = (int32_t)(4 * a);
+    while (lo < hi) {  // first column >= 4a
+      const int64_t mid = (lo + hi) >> 1;
+      if (cols[mid] < c0) lo = mid + 1;
+      else hi = mid;
+    }
+    for (int j = 0; j < 4; ++j) {
+      const int64_t k = lo + j;
+      M[i][j] = (k < rp[row + 1] && cols[k] == c0 + j) ? vals[k] : 0.0;
+      M[i][4 + j] = (i == j) ? 1.0 : 0.0;
+    }
+  }
+  bool ok = true;
+  for (int c = 0; c < 4; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (fabs(M[r][c]) > fabs(M[p][c])) p = r;
+    if (M[p][c] == 0.0) {
+      ok = false;
+      break;
+    }
+    if (p != c)
+      for (int j = 0; j < 8; ++j) {
+        const double t = M[c][j];
+        M[c][j] = M[p][j];
+        M[p][j] = t;
+      }
+    const double inv = 1.0 / M[c][c];
+    for (int j = 0; j < 8; ++j) M[c][j] *= inv;
+    for (int r = 0; r < 4; ++r)
+      if (r != c) {
+        const double f = M[r][c];
+        for (int j = 0; j < 8; ++j) M[r][j] -= f * M[c][j];
+      }
+  }
+  double* out = binv + 16 * a;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = ok ? M[i][4 + j] : (i == j ? 1.0 : 0.0);
+  if (!ok) atomicOr(status, 1);
+}
+
+// y = Binv x, node by node.
+__global__ __launch_bounds__(kSolveThreads) void block_apply_kernel(int64_t nn, const double* binv,
+                                                                    const double* x, double* y) {
+  const int64_t a = (int64_t)blockIdx.x * kSolveThreads + threadIdx.x;
+  if (a >= nn) return;
+  const double* B = binv + 16 * a;
+  const double x0 = x[4 * a], x1 = x[4 * a + 1], x2 = x[4 * a + 2], x3 = x[4 * a + 3];
+  for (int i = 0; i < 4; ++i) y[4 * a + i] = B[4 * i] * x0 + B[4 * i + 1] * x1 + B[4 * i + 2] * x2 + B[4 * i + 3] * x3;
+}
+
+// partial[j][block] = sum over the block's slice of V_j . w, j < k.
+__global__ __launch_bounds__(kSolveThreads) void multi_dot_partial_kernel(int64_t n, int k, const double* V,
+                                                                          int64_t ldv, const double* w,
+                                                                          double* partial) {
+  __shared__ double red[kSolveThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t stride = (int64_t)gridDim.x * kSolveThreads;
+  for (int j = 0; j < k; ++j) {
+    const double* v = V + (int64_t)j * ldv;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kSolveThreads + tid; i < n; i += stride) s += v[i] * w[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[wv] = s;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int q = 0; q < kSolveThreads / 64; ++q) t += red[q];
+      partial[(int64_t)j * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// out[j] = sum_b partial[j][b], one block per j.
+__global__ __launch_bounds__(kSolveThreads) void multi_dot_final_kernel(int nb, const double* partial, double* out) {
+  __shared__ double red[kSolveThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, j = blockIdx.x;
+  double s = 0.0;
+  for (int b = tid; b < nb; b += kSolveThreads) s += partial[(int64_t)j * nb + b];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kSolveThreads / 64; ++q) t += red[q];
+    out[j] = t;
+  }
+}
+
+// w += sign * sum_j c_j V_j, j < k.
+__global__ __launch_bounds__(kSolveThreads) void multi_axpy_kernel(int64_t n, int k, const double* V, int64_t ldv,
+                                                                   const double* c, double sign, double* w) {
+  const int64_t i = (int64_t)blockIdx.x * kSolveThreads + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < k; ++j) s += c[j] * V[(int64_t)j * ldv + i];
+  w[i] += sign * s;
+}
+
+// y = a * x  (y may alias x)
+__global__ __launch_bounds__(kSolveThreads) void scale_kernel(int64_t n, double a, const double* x, double* y) {
+  const int64_t i = (int64_t)blockIdx.x * kSolveThreads + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+// y = b - y
+__global__ __launch_bounds__(kSolveThreads) void rsub_kernel(int64_t n, const double* b, double* y) {
+  const int64_t i = (int64_t)blockIdx.x * kSolveThreads + threadIdx.x;
+  if (i < n) y[i] = b[i] - y[i];
+}
+
+dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kSolveThreads - 1) / kSolveThreads)); }
+
+}  // namespace
+
+int multi_dot_blocks(int64_t n) {
+  const int64_t b = (n + kSolveThreads - 1) / kSolveThreads;
+  return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 1024);
+}
+
+hipError_t launch_spmv(int64_t nrows, const int64_t* rp, const int32_t* cols, const double* vals, const double* x,
+                       double* y, hipStream_t s) {
+  if (nrows == 0) return hipSuccess;
+  const int64_t blocks = (nrows + kSolveThreads / 64 - 1) / (kSolveThreads / 64);
+  hipLaunchKernelGGL(spmv_kernel, dim3((unsigned)blocks), dim3(kSolveThreads), 0, s, nrows, rp, cols, vals, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_inverse(int64_t nn, const int64_t* rp, const int32_t* cols, const double* vals,
+                                double* binv, int* status, hipStream_t s) {
+  if (nn == 0) return hipSuccess;
+  hipLaunchKernelGGL(block_inverse_kernel, grid_for(nn), dim3(kSolveThreads), 0, s, nn, rp, cols, vals, binv, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_apply(int64_t nn, const double* binv, const double* x, double* y, hipStream_t s) {
+  if (nn == 0) return hipSuccess;
+  hipLaunchKernelGGL(block_apply_kernel, grid_for(nn), dim3(kSolveThreads), 0, s, nn, binv, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_multi_dot(int64_t n, int k, const double* V, int64_t ldv, const double* w, double* partial,
+                            double* out, hipStream_t s) {
+  if (k == 0) return hipSuccess;
+  const int nb = multi_dot_blocks(n);
+  hipLaunchKernelGGL(multi_dot_partial_kernel, dim3(nb), dim3(kSolveThreads), 0, s, n, k, V, ldv, w, partial);
+  hipLaunchKernelGGL(multi_dot_final_kernel, dim3(k), dim3(kSolveThreads), 0, s, nb, partial, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_multi_axpy(int64_t n, int k, const double* V, int64_t ldv, const double* c, double sign, double* w,
+                             hipStream_t s) {
+  if (n == 0 || k == 0) return hipSuccess;
+  hipLaunchKernelGGL(multi_axpy_kernel, grid_for(n), dim3(kSolveThreads), 0, s, n, k, V, ldv, c, sign, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(int64_t n, double a, const double* x, double* y, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(scale_kernel, grid_for(n), dim3(kSolveThreads), 0, s, n, a, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_rsub(int64_t n, const double* b, double* y, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rsub_kernel, grid_for(n), dim3(kSolveThreads), 0, s, n, b, y);
+  return hipGetLastError();
+}
+
+}  // namespace goal

@@ -47,6 +47,8 @@ int goal_memcpy_d2h(void* dst, const void* src_dev, int64_t bytes);
 int goal_sync(void);
 /* dst[i] = src[i] for i < n with the library's streaming copy kernel (async). */
 int goal_device_copy(double* dst_dev, const double* src_dev, int64_t n);
+/* p[i] = value for i < n (async; a fill kernel, so it may be graph-captured). */
+int goal_device_fill(double* p_dev, int64_t n, double value);
 /* HIP events on the library stream: record into slot 0..255, elapsed a->b in ms. */
 int goal_event_record(int slot);
 int goal_event_elapsed(int slot_a, int slot_b, float* ms);
@@ -146,6 +148,17 @@ int goal_solinfo_dev(goal_solinfo s, double** R_dev, double** dMdu_dev, double**
 int goal_solinfo_norm_R(goal_solinfo s, double* norm);
 /* -(R . z) (src/goal_nested_adjoint.cpp:213) with z a device vector */
 int goal_solinfo_dot_R(goal_solinfo s, const double* z_dev, double* dot);
+/* Linear solve dRdu x = b on the SolInfo's CSR values, standing in for
+ * goal::solve (src/goal_linear_solve.cpp:22-90: Belos block GMRES, block
+ * size 1, "krylov size" = restart, DGKS orthogonalization, left MueLu
+ * preconditioner).  Here: GMRES(restart) preconditioned on the right by the
+ * inverse 4x4 diagonal block of every node, classical Gram-Schmidt applied
+ * twice.  b, x are device vectors of num_dofs; x holds the initial guess on
+ * entry.  Stops when ||b - A x|| <= rtol ||b - A x0|| or after max_iters inner
+ * iterations; like the reference it returns normally when the limit is hit
+ * (*relres tells).  Outputs may be NULL. */
+int goal_linear_solve(goal_solinfo s, const double* b_dev, double* x_dev, double rtol, int max_iters, int restart,
+                      int* iters, double* relres);
 
 /* ------------------------------------------------------------- mechanics */
 typedef struct goal_material {

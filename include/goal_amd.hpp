@@ -402,6 +402,22 @@ inline void set_jac_dbcs(std::vector<DirichletBC> const& bcs, SolInfo* s, double
 // Return 0 on success.
 using LinearSolve = std::function<int(Disc* d, SolInfo* s, const double* b_dev, double* x_dev)>;
 
+// The library's device solve as a LinearSolve: GMRES(restart) with a node
+// block-Jacobi preconditioner from a zero initial guess, the parameters of
+// goal's "linear solve" list ("tolerance", "max iters", "krylov size";
+// src/goal_linear_solve.cpp:22-50).  Like the reference it does not fail when
+// the iteration limit is reached.
+struct GmresSolve {
+  double tolerance = 1e-10;
+  int max_iters = 1000;
+  int krylov_size = 100;
+  int operator()(Disc* d, SolInfo* s, const double* b_dev, double* x_dev) const {
+    int st = goal_device_fill(x_dev, d->get_num_dofs(), 0.0);
+    if (st != GOAL_OK) return st;
+    return goal_linear_solve(s->handle(), b_dev, x_dev, tolerance, max_iters, krylov_size, nullptr, nullptr);
+  }
+};
+
 struct PrimalParams {
   std::vector<DirichletBC> dirichlet;
   int max_iters = 5;       // "nonlinear max iters"

@@ -4,7 +4,7 @@
 // error estimate.  The linear solves use a dense LU, adequate for the small
 // meshes this program is run on.  Prints one JSON object.
 //
-// usage: surface_test <order> <quad_order> <model> <n> <steps> [tangent bits: 64 | 32]
+// usage: surface_test <order> <quad_order> <model> <n> <steps> [tangent bits: 64 | 32] [solver: lu | gmres]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -63,6 +63,7 @@ int main(int argc, char** argv) {
   const int order = std::atoi(argv[1]), quad = std::atoi(argv[2]), model = std::atoi(argv[3]);
   const int n = std::atoi(argv[4]), steps = std::atoi(argv[5]);
   const int tangent = argc > 6 ? std::atoi(argv[6]) : 64;
+  const bool gmres = argc > 7 && std::string(argv[7]) == "gmres";
   try {
     auto disc = goal::Disc::box(n, n, n, 1.0, 1.0, 1.0, order, 0.1, 11);
     goal::MechanicsParams mp;
@@ -78,7 +79,11 @@ int main(int argc, char** argv) {
     pp.dirichlet = {{0, "zmin", "0.0"}, {1, "zmin", "0.0"}, {2, "zmin", "0.0"}, {2, "zmax", "0.01*t"}};
     pp.max_iters = 10;
     pp.tolerance = 1e-10;
-    goal::Primal primal(pp, &mech, dense_solve);
+    goal::GmresSolve gm;
+    gm.tolerance = 1e-12;
+    gm.max_iters = 5000;
+    const goal::LinearSolve solve = gmres ? goal::LinearSolve(gm) : goal::LinearSolve(dense_solve);
+    goal::Primal primal(pp, &mech, solve);
     primal.build_data();
     goal::Functional functional("avg disp", -1, &primal);
     std::printf("{\"iters\": [");
@@ -95,7 +100,7 @@ int main(int argc, char** argv) {
     }
     std::printf("], \"J\": %.17g, \"norm\": %.6e", J, primal.last_norm());
     if (order == 1) {
-      goal::NestedAdjoint adj(&mech, "avg disp", -1, pp.dirichlet, dense_solve);
+      goal::NestedAdjoint adj(&mech, "avg disp", -1, pp.dirichlet, solve);
       std::vector<double> eta = adj.run(t_old, t_old - 1.0);
       double s = 0.0;
       for (double e : eta) s += e;

@@ -31,13 +31,16 @@ def csr(m, vals):
     return sp.csr_matrix((vals, m.cols, m.row_ptr), shape=(m.num_dofs, m.num_dofs))
 
 
-def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10):
-    coords, conn = g.mesh_box(n, n, n, order=order, jitter=0.1, seed=11)
+def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0.1, seed=11, dwr=True,
+                 threads=1):
+    """Oracle Newton load steps (and, for P1, one nested-adjoint estimate)."""
+    coords, conn = g.mesh_box(n, n, n, order=order, jitter=jitter, seed=seed)
     m = O.Mesh(order, coords, conn)
     mats = [NH_ROW if model == 1 else J2_ROW]
     st = O.States(m, model, quad)
     x = np.zeros(m.num_dofs)
     iters = []
+    Js = []
     t_old = 0.0
     J = 0.0
     norm = 0.0
@@ -46,13 +49,13 @@ def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10):
         it = 1
         converged = False
         while it <= max_iters and not converged:
-            R, V = O.assemble(O.K_JACOBIAN, m, model, mats, x, st, quad_order=quad)
+            R, V = O.assemble(O.K_JACOBIAN, m, model, mats, x, st, quad_order=quad, threads=threads)
             dM = np.zeros(m.num_dofs)
             for comp, nodes, vals in dbcs(coords, t):
                 O.dbc(m, nodes, vals, comp, x, R, dM, V)
             du = spla.spsolve(csr(m, V).tocsc(), -R)
             x = x + du
-            R, _ = O.assemble(O.K_RESIDUAL, m, model, mats, x, st, quad_order=quad)
+            R, _ = O.assemble(O.K_RESIDUAL, m, model, mats, x, st, quad_order=quad, threads=threads)
             for comp, nodes, vals in dbcs(coords, t):
                 O.dbc(m, nodes, vals, comp, x, R)
             norm = np.linalg.norm(R)
@@ -62,11 +65,12 @@ def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10):
             raise RuntimeError("newton failed")
         iters.append(it - 1)
         J, _, _ = O.qoi(0, m, model, mats, x, st, quad_order=quad)
+        Js.append(J)
         if step < steps:
             st.update()
         t_old = t
-    out = {"iters": iters, "J": J, "norm": norm}
-    if order == 1:
+    out = {"iters": iters, "J": J, "J_steps": Js, "norm": norm}
+    if order == 1 and dwr:
         out.update(nested_estimate(coords, conn, x, st, model, mats, quad, t_old))
     return out
 
@@ -103,3 +107,52 @@ def nested_estimate(coords, conn, xb, st_base, model, mats, quad, t):
     eta = O.compute_error(m, e)
     base = O.set_error(parent, eta, conn.shape[0])
     return {"eta_sum": float(base.sum()), "bound": float(bound), "zR": float(zR)}
+
+
+def gpu_newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0.1, seed=11, tangent=64,
+                     solver="device", lin_rtol=1e-12):
+    """The same load steps with the assembly, Dirichlet rows and norms on the
+    GPU (through the C ABI); the linear solve is the library's device GMRES
+    (solver="device") or scipy's direct solve on the host (solver="scipy")."""
+    disc = g.Disc.box(n, n, n, order=order, jitter=jitter, seed=seed)
+    mat = dict(zip(("E", "nu", "K", "Y", "c0"), NH_ROW if model == 1 else J2_ROW))
+    st = g.States(disc, model, quad)
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=model, **mat)], stabilization=True, quad_order=quad)
+    mech.set_option("tangent", tangent)
+    rp, cols = disc.graph()
+    si = g.SolInfo(disc)
+    bcs = [(0, "zmin", "0.0"), (1, "zmin", "0.0"), (2, "zmin", "0.0"), (2, "zmax", "0.01*t")]
+    iters, Js, lin = [], [], []
+    norm = 0.0
+    b, du = g.DeviceArray(disc.num_dofs), g.DeviceArray(disc.num_dofs)
+    for step in range(1, steps + 1):
+        t = float(step)
+        it, converged = 1, False
+        while it <= max_iters and not converged:
+            si.zero_all()
+            mech.jacobian(si, adjoint=False, save_states=True)
+            for comp, ns, expr in bcs:
+                g.dbc_apply(disc, si, comp, ns, expr, t, True)
+            if solver == "device":
+                g.check(g.lib.goal_memcpy_h2d(b.ptr, g._ptr(np.ascontiguousarray(-si.R())), 8 * disc.num_dofs))
+                du.fill(0.0)
+                k, rr = si.solve(b, du, rtol=lin_rtol, max_iters=5000, restart=100)
+                lin.append(k)
+                disc.add_soln_dev(du)
+            else:
+                A = sp.csr_matrix((si.vals(), cols, rp), shape=(disc.num_dofs, disc.num_dofs))
+                disc.add_soln(spla.spsolve(A.tocsc(), -si.R()))
+            si.zero_R()
+            mech.residual(si, save_states=True)
+            for comp, ns, expr in bcs:
+                g.dbc_apply(disc, si, comp, ns, expr, t, False)
+            norm = si.norm_R()
+            converged = norm < tol
+            it += 1
+        if not converged:
+            raise RuntimeError("newton failed")
+        iters.append(it - 1)
+        Js.append(mech.qoi(0))
+        if step < steps:
+            st.update()
+    return {"iters": iters, "J": Js[-1], "J_steps": Js, "norm": norm, "linear_iters": lin}

@@ -8,6 +8,7 @@ import json
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from tests import drivers
@@ -17,10 +18,10 @@ pytestmark = pytest.mark.gpu
 EXE = Path(__file__).resolve().parent / "cpp" / "surface_test"
 
 
-def run_surface(order, quad, model, n, steps, tangent=64):
+def run_surface(order, quad, model, n, steps, tangent=64, solver="lu"):
     if not EXE.exists():
         pytest.fail(f"{EXE} is not built: run __graft_entry__.build()")
-    r = subprocess.run([str(EXE), str(order), str(quad), str(model), str(n), str(steps), str(tangent)],
+    r = subprocess.run([str(EXE), str(order), str(quad), str(model), str(n), str(steps), str(tangent), solver],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     return json.loads(r.stdout)
@@ -62,3 +63,37 @@ def test_fp32_tangent_newton(order, quad, model, n, steps):
         # z.R sums products of opposite sign (J2 case: 4e-7 out of terms ~1e-3),
         # which amplifies the adjoint's FP32 rounding: 1e-4 relative
         assert abs(f32["zR"] - f64["zR"]) <= 1e-4 * abs(f64["zR"])
+
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_cfg1_newton_matches_oracle():
+    """The CPU-runnable workload end to end: 8^3 hexahedra -> 3,072 P2 tets,
+    neo-Hookean, 4-point quadrature, ten load steps to u_z = 0.1 on z = 1,
+    Newton to ||R|| < 1e-10 with the device GMRES.  Same iteration count per
+    load step as the oracle with a direct solve (tests/golden/cfg1_newton.json),
+    the average displacement within 1e-9 relative at every step."""
+    ref = json.loads((GOLDEN / "cfg1_newton.json").read_text())
+    c = ref["config"]
+    gpu = drivers.gpu_newton_steps(c["order"], c["quad"], c["model"], c["n"], c["steps"], jitter=c["jitter"],
+                                   seed=c["seed"])
+    print("newton iterations", gpu["iters"], "linear iterations", gpu["linear_iters"])
+    assert gpu["iters"] == ref["iters"]
+    assert gpu["norm"] < 1e-10
+    for a, b in zip(gpu["J_steps"], ref["J_steps"]):
+        assert abs(a - b) <= 1e-9 * abs(b)
+
+
+@pytest.mark.parametrize("order,quad,model,n,steps", [(1, 1, 1, 3, 3), (2, 2, 1, 2, 2)])
+def test_cpp_surface_with_device_gmres(order, quad, model, n, steps):
+    """goal::GmresSolve as the LinearSolve of Primal and NestedAdjoint: the
+    same Newton iteration counts and converged quantities as the dense LU."""
+    lu = run_surface(order, quad, model, n, steps)
+    gm = run_surface(order, quad, model, n, steps, solver="gmres")
+    assert gm["iters"] == lu["iters"]
+    assert gm["norm"] < 1e-10
+    assert abs(gm["J"] - lu["J"]) <= 1e-9 * abs(lu["J"])
+    if order == 1:
+        for k in ("eta_sum", "bound", "zR"):
+            assert abs(gm[k] - lu[k]) <= 1e-7 * abs(lu[k]), k

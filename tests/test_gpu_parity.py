@@ -272,3 +272,36 @@ def test_accumulate_without_zero():
     mech.jacobian(si, save_states=False)
     assert rel(si.R(), 2 * R1) < 1e-15
     assert rel(si.vals(), 2 * V1) < 1e-15
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_linear_solve(order):
+    """Device GMRES (node block-Jacobi, left preconditioned) on an assembled
+    Jacobian with Dirichlet rows, against scipy's direct solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    disc, st, mech, x, om, ost, mats = make(order, 2 if order == 2 else 1, 1)
+    # a physical iterate near the reference configuration (the random one can
+    # invert elements, which no Newton step of the mechanics problem solves at)
+    disc.set_soln(1e-4 * random_state(disc.num_nodes, 5))
+    si = g.SolInfo(disc)
+    si.zero_all()
+    mech.jacobian(si)
+    for comp in range(3):
+        g.dbc_apply(disc, si, comp, "zmin", "0.0", 0.0, True)
+    rp, cols = disc.graph()
+    A = sp.csr_matrix((si.vals(), cols, rp), shape=(disc.num_dofs, disc.num_dofs))
+    rhs = np.random.default_rng(3).standard_normal(disc.num_dofs)
+    ref = spla.spsolve(A.tocsc(), rhs)
+    b, y = g.DeviceArray.from_host(rhs), g.DeviceArray(disc.num_dofs)
+    y.fill(0.0)
+    it, rr = si.solve(b, y, rtol=1e-12, max_iters=5000, restart=100)
+    out = y.numpy()
+    assert rr <= 1e-12 and it > 0
+    assert np.linalg.norm(A @ out - rhs) <= 1e-9 * np.linalg.norm(rhs)
+    assert rel(out, ref) < 1e-8
+    # restarting from the solution keeps it (the stopping test is relative to
+    # the initial residual, so this runs to the iteration limit)
+    si.solve(b, y, rtol=1e-10, max_iters=60, restart=30)
+    assert rel(y.numpy(), ref) < 1e-8
