@@ -33,12 +33,47 @@ ASSEMBLY_KERNELS = ("element_kernel", "gather_kernel")
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def profile_tag(n: int, order: int, tangent: int = 64) -> str:
-    return f"assembly_p{order}_nh_cube{n}" + ("_f32tangent" if tangent == 32 else "")
+NH_MAT = dict(E=1000.0, nu=0.25, K=0.0, Y=0.0, c0=1.0)
+J2_MAT = dict(E=200.0, nu=0.3, K=2.0, Y=0.25, c0=1.0)
+B_STATE_J2 = 232  # J2 history per stored point: read Fp_old + eqps_old, write Fp + eqps + sigma
+
+
+def workload(name: str, n: int, order: int):
+    """Mesh, material and iterate of a benchmark workload."""
+    if name == "cube":
+        return dict(name=name, dims=(n, n, n), lengths=(1.0, 1.0, 1.0), order=order, quad=2, model=1, mat=NH_MAT,
+                    b_elem=B_ELEM_P2 if order == 2 else B_ELEM_P1, npts=4,
+                    metric=f"P{order}-tet resid+Jac assembled elems/s",
+                    desc=f"cube {n}^3 hex, jitter 10% h, neo-Hookean E=1000 nu=0.25 c0=1, 4-pt quadrature")
+    if name == "bar":
+        return dict(name=name, dims=(384, 48, 48), lengths=(8.0, 1.0, 1.0), order=1, quad=1, model=2, mat=J2_MAT,
+                    b_elem=B_ELEM_P1 + B_STATE_J2, npts=1, metric="P1-tet J2 resid+Jac assembled elems/s",
+                    desc="bar [0,8]x[0,1]^2 as 384x48x48 hex, jitter 10% h, J2 E=200 nu=0.3 Y=0.25 K=2 c0=1, "
+                         "1-pt quadrature, FP64 per-point history, iterate at 1% axial strain (plastic)")
+    raise ValueError(name)
+
+
+def profile_tag(n: int, order: int, tangent: int = 64, name: str = "cube") -> str:
+    base = f"assembly_p{order}_nh_cube{n}" if name == "cube" else "assembly_p1_j2_bar384"
+    return base + ("_f32tangent" if tangent == 32 else "")
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bar_state(coords):
+    """Uniaxial stretch of 1% along x (yield strain 1.25e-3), a smooth transverse
+    perturbation, and the matching bulk pressure."""
+    import numpy as np
+
+    X, Y, Z = coords[:, 0], coords[:, 1], coords[:, 2]
+    x = np.zeros(4 * coords.shape[0])
+    x[0::4] = 0.01 * X
+    x[1::4] = -0.003 * Y + 1e-3 * np.sin(np.pi * X / 8) * Z
+    x[2::4] = -0.003 * Z + 1e-3 * np.cos(np.pi * X / 8) * Y
+    x[3::4] = 0.5
+    return x
 
 
 def smooth_state(coords, amp=0.02):
@@ -53,18 +88,27 @@ def smooth_state(coords, amp=0.02):
     return x
 
 
-def cpu_baseline(n_sample: int, threads: int, order: int, quad: int, reps: int = 5):
-    """Oracle (C++ restatement, dual numbers, column-search scatter) on a sample."""
+def cpu_baseline(n_sample: int, threads: int, wl, reps: int = 5):
+    """Oracle (C++ restatement, dual numbers, column-search scatter) on a sample
+    of the workload: n_sample^3 hexahedra of the cube, or the first 2 n_sample
+    hexahedron layers of the bar's full 48 x 48 cross-section (same cell size)."""
     import numpy as np
 
     import goal_amd as g
     import oracle as O
 
-    coords, conn = g.mesh_box(n_sample, n_sample, n_sample, order=order, jitter=0.1, seed=0x5EED)
+    order, quad = wl["order"], wl["quad"]
+    if wl["name"] == "cube":
+        coords, conn = g.mesh_box(n_sample, n_sample, n_sample, order=order, jitter=0.1, seed=0x5EED)
+        x = smooth_state(coords)
+    else:
+        k = max(1, 2 * n_sample)
+        coords, conn = g.mesh_box(k, 48, 48, k / 48.0, 1.0, 1.0, order=order, jitter=0.1, seed=0x5EED)
+        x = bar_state(coords)
     m = O.Mesh(order, coords, conn)
-    st = O.States(m, 1, quad)
-    x = smooth_state(coords)
-    mats = [[1000.0, 0.25, 0.0, 0.0, 1.0]]
+    st = O.States(m, wl["model"], quad)
+    mt = wl["mat"]
+    mats = [[mt["E"], mt["nu"], mt["K"], mt["Y"], mt["c0"]]]
     R = np.zeros(m.num_dofs)
     V = np.zeros(m.nnz)
     times = []
@@ -72,7 +116,7 @@ def cpu_baseline(n_sample: int, threads: int, order: int, quad: int, reps: int =
         R[:] = 0.0
         V[:] = 0.0
         t0 = time.perf_counter()
-        O.assemble(O.K_JACOBIAN, m, 1, mats, x, st, quad_order=quad, R=R, vals=V, threads=threads)
+        O.assemble(O.K_JACOBIAN, m, wl["model"], mats, x, st, quad_order=quad, R=R, vals=V, threads=threads)
         dt = time.perf_counter() - t0
         if k > 0:
             times.append(dt)
@@ -190,6 +234,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="hexahedra per cube side")
     ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--workload", default="cube", choices=("cube", "bar"),
+                    help="cube: P2 neo-Hookean cube (the headline); bar: P1 J2 bar, 5.3 M tets")
     ap.add_argument("--blocks", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="hexahedra per side of the CPU baseline sample")
@@ -202,14 +248,15 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.summarize:
-        base = f"python3 bench.py --n {args.n} --order {args.order} --tangent {args.tangent}"
+        base = f"python3 bench.py --workload {args.workload} --n {args.n} --order {args.order} --tangent {args.tangent}"
         cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d {args.summarize[0]} -o run -- "
                 f"{base} --steps 5 --warmup 1 --no-cpu-baseline",
                 f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {args.summarize[1]} -o run -- "
                 f"{base} --steps 2 --warmup 1 --no-cpu-baseline",
                 f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {args.summarize[2]} -o run -- "
                 f"{base} --steps 2 --warmup 1 --no-cpu-baseline"]
-        return summarize(*args.summarize, profile_tag(args.n, args.order, args.tangent), cmds, args.tangent)
+        return summarize(*args.summarize, profile_tag(args.n, args.order, args.tangent, args.workload), cmds,
+                         args.tangent)
 
     if args.gpus != 1:
         print(
@@ -225,22 +272,23 @@ def main() -> int:
 
     name, cus = g.device_info()
     log(f"device: {name}, {cus} CUs")
-    order, quad = args.order, 2
+    wl = workload(args.workload, args.n, args.order)
+    order, quad = wl["order"], wl["quad"]
     n = args.n
     t0 = time.perf_counter()
-    disc = g.Disc.box(n, n, n, order=order, jitter=0.1, seed=0x5EED)
+    disc = g.Disc.box(*wl["dims"], *wl["lengths"], order=order, jitter=0.1, seed=0x5EED)
     log(f"mesh+graph: {disc.num_elems} tets, {disc.num_nodes} nodes, {disc.num_dofs} dofs, nnz {disc.nnz}, "
         f"max degree {disc.max_degree} ({time.perf_counter() - t0:.1f} s)")
-    st = g.States(disc, g.MODEL_NEOHOOKEAN, quad)
-    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=g.MODEL_NEOHOOKEAN, E=1000.0, nu=0.25, c0=1.0)],
-                       stabilization=True, quad_order=quad)
+    st = g.States(disc, wl["model"], quad)
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=wl["model"], **wl["mat"])], stabilization=True,
+                       quad_order=quad)
     mech.set_option("check", 0)
     if args.blocks:
         mech.set_option("blocks", args.blocks)
     mech.set_option("xcd_remap", args.xcd_remap)
     mech.set_option("tangent", args.tangent)
     coords, _, _ = disc.mesh()
-    disc.set_soln(smooth_state(coords))
+    disc.set_soln(smooth_state(coords) if wl["name"] == "cube" else bar_state(coords))
     si = g.SolInfo(disc)
 
     def step(slot=None):
@@ -321,9 +369,9 @@ def main() -> int:
     del a, b
     log(f"copy kernel: {copy_gbs:.0f} GB/s")
 
-    b_elem = B_ELEM_P2 if order == 2 else B_ELEM_P1
+    b_elem = wl["b_elem"]
     achieved = b_elem * disc.num_elems / (kernel_ms * 1e-3) / 1e9 if kernel_ms else None
-    traffic, traffic_src = load_traffic(profile_tag(n, order, args.tangent), element_ms, gather_ms)
+    traffic, traffic_src = load_traffic(profile_tag(n, order, args.tangent, wl["name"]), element_ms, gather_ms)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1) if achieved else None,
@@ -337,20 +385,23 @@ def main() -> int:
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {args.cpu_sample}^3-hex sample, {threads} threads")
-        v, ne_s, med = cpu_baseline(args.cpu_sample, threads, order, quad)
+        v, ne_s, med = cpu_baseline(args.cpu_sample, threads, wl)
         cpu = {
             "value": round(v, 1),
             "unit": "elem/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"{ne_s} P2 tets ({args.cpu_sample}^3 hex cube, same jitter/material), "
-                      f"residual+Jacobian with FAD-40 and column-search CSR scatter, median of 5 after 1 warm-up "
+            "sample": f"{ne_s} P{order} tets ("
+                      + (f"{args.cpu_sample}^3 hex cube" if wl["name"] == "cube" else
+                         f"first {2 * args.cpu_sample} of the bar's 384 hex layers") +
+                      f", same jitter/material/iterate), residual+Jacobian with "
+                      f"FAD-{16 if order == 1 else 40} and column-search CSR scatter, median of 5 after 1 warm-up "
                       f"({med:.2f} s each), host {platform.processor() or platform.machine()}",
         }
         log(f"cpu baseline: {v:.0f} elem/s ({elems_per_s / v:.0f}x)")
 
     out = {
-        "metric": "P2-tet resid+Jac assembled elems/s",
+        "metric": wl["metric"],
         "value": round(elems_per_s, 1),
         "unit": "elem/s",
         "n_gpus": 1,
@@ -361,14 +412,13 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.tangent == 64 else "f64 residual, f32 element matrices",
-        "data": "synthetic (Kuhn-tet cube mesh, smooth synthetic displacement/pressure iterate)",
+        "data": "synthetic (Kuhn-tet box mesh, smooth synthetic displacement/pressure iterate)",
         "config": {
-            "workload": f"cube {n}^3 hex -> {disc.num_elems} P2 tets, {disc.num_dofs} dofs, nnz {disc.nnz}, "
-                        f"jitter 10% h, neo-Hookean E=1000 nu=0.25 c0=1, 4-pt quadrature, FP64 residual + "
-                        f"Jacobian (zero_all + assembly)",
+            "workload": f"{wl['desc']} -> {disc.num_elems} P{order} tets, {disc.num_dofs} dofs, nnz {disc.nnz}; "
+                        f"residual + Jacobian (zero_all + assembly)",
             "elements": disc.num_elems,
             "order": order,
-            "quad_points": 4,
+            "quad_points": wl["npts"],
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
