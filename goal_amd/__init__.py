@@ -136,6 +136,11 @@ _sig("goal_asm_localize", _i32, _p, _p, _p, _p)
 _sig("goal_eval", _i32, C.c_char_p, _d, _d, _d, _d, _dp)
 _sig("goal_dbc_apply", _i32, _p, _p, _i32, C.c_char_p, C.c_char_p, _d, _i32)
 _sig("goal_dbc_apply_values", _i32, _p, _p, _i32, _i64, _p, _p, _i32)
+_sig("goal_batch_create", _i32, _i32, _p, _p, C.POINTER(_p))
+_sig("goal_batch_destroy", _i32, _p)
+_sig("goal_batch_tune", _i32, _p, _i32, _p, _p)
+_sig("goal_batch_capture", _i32, _p)
+_sig("goal_batch_run", _i32, _p)
 _sig("goal_nested_create", _i32, _p, C.POINTER(_p))
 _sig("goal_nested_destroy", _i32, _p)
 _sig("goal_nested_disc", _p, _p)
@@ -425,6 +430,10 @@ class SolInfo:
         check(lib.goal_solinfo_get(self.h, None, None, _ptr(out)))
         return out
 
+    def set(self, R=None, dMdu=None, vals=None) -> None:
+        conv = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (R, dMdu, vals)]
+        check(lib.goal_solinfo_set(self.h, *[_ptr(a) for a in conv]))
+
     def dev(self):
         r, m, v = _p(), _p(), _p()
         check(lib.goal_solinfo_dev(self.h, C.byref(r), C.byref(m), C.byref(v)))
@@ -509,6 +518,36 @@ class Mechanics:
 
     def localize(self, s: SolInfo, z_diff_dev, z_coarse_dev) -> None:
         check(lib.goal_asm_localize(self.h, s.h, z_diff_dev, z_coarse_dev))
+
+
+class Batch:
+    """Jacobian-mode assemblies of many (Mechanics, SolInfo) pairs as one
+    captured HIP graph (goal_batch_*)."""
+
+    def __init__(self, mechs, infos):
+        self.mechs, self.infos = list(mechs), list(infos)
+        self.h = _p()
+        mh = (_p * len(self.mechs))(*[m.h.value for m in self.mechs])
+        sh = (_p * len(self.infos))(*[s.h.value for s in self.infos])
+        check(lib.goal_batch_create(len(self.mechs), mh, sh, C.byref(self.h)))
+
+    def tune(self, reps: int = 3):
+        """Autotune each mesh's grid; returns (blocks, median ms) per mesh."""
+        blocks = np.zeros(len(self.mechs), dtype=np.int32)
+        ms = np.zeros(len(self.mechs), dtype=np.float32)
+        check(lib.goal_batch_tune(self.h, reps, _ptr(blocks), _ptr(ms)))
+        return blocks, ms
+
+    def capture(self) -> None:
+        check(lib.goal_batch_capture(self.h))
+
+    def run(self) -> None:
+        check(lib.goal_batch_run(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.goal_batch_destroy(self.h)
+            self.h = _p()
 
 
 def dbc_apply(disc: Disc, s: SolInfo, comp: int, node_set: str, expr: str, t: float, jacobian: bool) -> None:

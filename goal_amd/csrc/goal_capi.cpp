@@ -207,6 +207,19 @@ struct goal_mech_s {
   }
 };
 
+// Batched assembly: many (mechanics, sol info) pairs captured into one HIP
+// graph, so a whole batch of meshes is one launch.
+struct goal_batch_s {
+  std::vector<goal_mech> mechs;
+  std::vector<goal_solinfo> infos;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~goal_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 struct goal_nested_s {
   goal_disc base = nullptr;
   goal_disc fine = nullptr;
@@ -1144,6 +1157,139 @@ int goal_asm_jacobian(goal_mech m, goal_solinfo s, int adjoint, int save_states)
     a.save_state = save_states ? 1 : 0;
     run_assembly(m, s, adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN, a, true);
     return finish_assembly(m);
+  });
+}
+
+int goal_batch_create(int n, const goal_mech* mechs, const goal_solinfo* infos, goal_batch* out) {
+  return guard([&]() -> int {
+    require(n >= 1 && mechs && infos && out, "goal_batch_create: bad arguments");
+    auto b = std::make_unique<goal_batch_s>();
+    for (int i = 0; i < n; ++i) {
+      require(mechs[i] && infos[i] && infos[i]->d == mechs[i]->d, "goal_batch_create: mismatched pair");
+      b->mechs.push_back(mechs[i]);
+      b->infos.push_back(infos[i]);
+    }
+    *out = b.release();
+    return GOAL_OK;
+  });
+}
+
+int goal_batch_destroy(goal_batch b) {
+  return guard([&]() -> int {
+    if (b) {
+      sync();
+      delete b;
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_batch_tune(goal_batch b, int reps, int* blocks_out, float* ms_out) {
+  return guard([&]() -> int {
+    require(b && reps >= 1, "goal_batch_tune: bad arguments");
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx().device);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto time_one = [&](goal_mech m, goal_solinfo s) {
+      std::vector<float> t;
+      for (int r = 0; r < reps + 1; ++r) {
+        s->zero_R = s->zero_dMdu = s->zero_vals = true;
+        goal::AsmArgs a = asm_args(m);
+        a.save_state = 1;
+        HIPCHK(hipEventRecord(e0, stream()));
+        run_assembly(m, s, goal::ASM_JACOBIAN, a, true);
+        HIPCHK(hipEventRecord(e1, stream()));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0) t.push_back(ms);  // the first run allocates and warms up
+      }
+      std::sort(t.begin(), t.end());
+      return t[t.size() / 2];
+    };
+    for (size_t i = 0; i < b->mechs.size(); ++i) {
+      goal_mech m = b->mechs[i];
+      // candidate persistent grids: the occupancy-sized one (0) and fixed
+      // multiples of the CU count; more workgroups than element groups are
+      // never useful
+      const int64_t ne = m->d->mesh.num_elems;
+      std::vector<int> cand{0};
+      for (int k : {1, 2, 4, 8, 16})
+        if ((int64_t)k * cus <= std::max<int64_t>(ne / 4, 8)) cand.push_back(k * cus);
+      int best = 0;
+      float best_ms = 1e30f;
+      for (int c : cand) {
+        m->blocks = c;
+        const float ms = time_one(m, b->infos[i]);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+      m->blocks = best;
+      if (blocks_out) blocks_out[i] = best;
+      if (ms_out) ms_out[i] = best_ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (b->exec) {  // launch shapes changed: capture again before the next run
+      (void)hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+    if (b->graph) {
+      (void)hipGraphDestroy(b->graph);
+      b->graph = nullptr;
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_batch_capture(goal_batch b) {
+  return guard([&]() -> int {
+    require(b, "goal_batch_capture: bad arguments");
+    for (size_t i = 0; i < b->mechs.size(); ++i) {
+      goal_mech m = b->mechs[i];
+      require(!m->timing, "goal_batch_capture: turn the \"timing\" option off first");
+      // one eager assembly sizes the element buffers (no allocation inside a capture)
+      b->infos[i]->zero_R = b->infos[i]->zero_dMdu = b->infos[i]->zero_vals = true;
+      goal::AsmArgs a = asm_args(m);
+      a.save_state = 1;
+      run_assembly(m, b->infos[i], goal::ASM_JACOBIAN, a, true);
+    }
+    sync();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+    try {
+      for (size_t i = 0; i < b->mechs.size(); ++i) {
+        // pending zero: the captured kernels overwrite R and the CSR values,
+        // so every replay assembles from scratch
+        b->infos[i]->zero_R = b->infos[i]->zero_dMdu = b->infos[i]->zero_vals = true;
+        goal::AsmArgs a = asm_args(b->mechs[i]);
+        a.save_state = 1;
+        run_assembly(b->mechs[i], b->infos[i], goal::ASM_JACOBIAN, a, true);
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream(), &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream(), &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return GOAL_OK;
+  });
+}
+
+int goal_batch_run(goal_batch b) {
+  return guard([&]() -> int {
+    require(b && b->exec, "goal_batch_run: capture the batch first");
+    HIPCHK(hipGraphLaunch(b->exec, stream()));
+    return GOAL_OK;
   });
 }
 

@@ -224,6 +224,27 @@ int goal_dbc_apply(goal_disc d, goal_solinfo s, int comp, const char* node_set, 
 int goal_dbc_apply_values(goal_disc d, goal_solinfo s, int comp, int64_t n, const int32_t* nodes,
                           const double* values, int jacobian);
 
+/* --------------------------------------------------------- batched assembly */
+/* Many meshes assembled as one launch: the Jacobian-mode assemblies of n
+ * (mechanics, sol info) pairs -- each what goal_asm_jacobian(m, s, 0, 1) does
+ * after goal_solinfo_zero_all(s) -- captured into one HIP graph.  No
+ * counterpart in the reference (one Mechanics per process there); it is for
+ * callers that assemble many small meshes. */
+typedef struct goal_batch_s* goal_batch;
+int goal_batch_create(int n, const goal_mech* mechs, const goal_solinfo* infos, goal_batch* out);
+int goal_batch_destroy(goal_batch b);
+/* Autotuner: for each mesh, time `reps` assemblies with each candidate
+ * persistent-grid size (the occupancy-sized grid and multiples of the CU
+ * count) and keep the fastest as its "blocks" option.  Outputs (may be NULL):
+ * the chosen grid (0 = occupancy-sized) and its median time in ms per mesh.
+ * A captured graph is dropped (capture again). */
+int goal_batch_tune(goal_batch b, int reps, int* blocks, float* ms);
+/* Capture the batch (one eager assembly per mesh first sizes the buffers).
+ * Every mesh's "timing" option must be off. */
+int goal_batch_capture(goal_batch b);
+/* Launch the captured graph on the library stream (async). */
+int goal_batch_run(goal_batch b);
+
 /* ------------------------------------------------------------ nested DWR */
 typedef struct goal_nested_s* goal_nested;
 /* Nested(Disc*, UNIFORM) (src/goal_nested.cpp:13-28, 144-154): uniform
