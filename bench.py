@@ -21,6 +21,7 @@ import platform
 import statistics
 import sys
 import time
+from pathlib import Path
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -322,6 +323,103 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_learned(args) -> int:
+    """Goal-oriented adaptation: learned indicator vs plain nested DWR."""
+    import numpy as np
+
+    import goal_amd as g
+    from goal_amd import learned as LP
+
+    name, _ = g.device_info()
+    path = Path(args.model) if args.model else LP.DEFAULT_MODEL
+    train_info = None
+    if args.retrain or not path.exists():
+        t0 = time.perf_counter()
+        ckpt = Path("gpurun_out") / "learned_dataset.npz" if Path("gpurun_out").is_dir() else None
+        X, Y, G, runs = LP.make_dataset(args.train_runs, seed=0, checkpoint=ckpt, log=log)
+        data_s = time.perf_counter() - t0
+        log(f"dataset: {X.shape[0]} samples from {args.train_runs} runs ({len(runs)} meshes) in {data_s:.1f} s")
+        net, stats = LP.train(X, Y, epochs=40, batch=4096, lr=2e-3, seed=0, groups=G, log=log)
+        train_info = dict(stats, dataset_seconds=round(data_s, 1), runs=args.train_runs, meshes=len(runs),
+                          samples=int(X.shape[0]))
+        out_path = Path(args.save_model) if args.save_model else Path("gpurun_out") / "learned_avgvm.npz"
+        out_path.parent.mkdir(parents=True, exist_ok=True)
+        LP.save(net, out_path, meta=train_info)
+        with open(out_path.with_suffix(".runs.json"), "w") as f:
+            json.dump(runs, f)
+        log(f"saved {out_path}")
+        if args.train_only:
+            print(json.dumps({"training": train_info, "model": str(out_path)}), flush=True)
+            return 0
+    else:
+        net = LP.load(path)
+        train_info = dict(net.meta, model=str(path.name))
+    rng = np.random.default_rng(args.eval_seed)
+    problems = [(LP.sample_problem(rng, "cube", 8), args.final_elems),
+                (LP.sample_problem(rng, "bar", 4), args.final_elems_bar)]
+    report = []
+    methods = ("nested", "learned", "uniform")
+    for p, final in problems:
+        log(f"problem: {p}, final {final} tets")
+        res = {}
+        for method in methods:
+            recs, last = LP.run_cycles(p, method, args.cycles, final, net=net, log=log)
+            res[method] = recs
+            if method == "nested":
+                ref = LP.overkill_qoi(last, log=log)
+            del last
+        for method in methods:
+            for r in res[method]:
+                err = abs(ref["J"] - r["J"])
+                r["qoi_error"] = err
+                if "eta_sum" in r and method != "uniform":
+                    r["effectivity"] = r["eta_sum"] / err if err > 0 else None
+                if "learned_eta_sum" in r:
+                    r["learned_effectivity_same_mesh"] = r["learned_eta_sum"] / err if err > 0 else None
+        report.append({"problem": p.__dict__, "final_elems": final, "reference": ref, **res})
+    est_n = sum(r["estimate_seconds"] for rep in report for r in rep["nested"] if "estimate_seconds" in r)
+    est_l = sum(r["estimate_seconds"] for rep in report for r in rep["learned"] if "estimate_seconds" in r)
+    # effectivities of the indicator sums on the same (nested-adapted) meshes
+    same = [(r["effectivity"], r["learned_effectivity_same_mesh"]) for rep in report for r in rep["nested"]
+            if r.get("effectivity") and r.get("learned_effectivity_same_mesh")]
+    gaps = [abs(a - b) for a, b in same]
+    rel = [abs(b / a - 1.0) for a, b in same]
+    fallbacks = sum(1 for rep in report for r in rep["learned"] if r.get("fallback"))
+    final_err = {rep["problem"]["geometry"]: {m: rep[m][-1]["qoi_error"] for m in methods} for rep in report}
+    out = {
+        "metric": "learned vs plain nested-DWR error estimation speed-up (AvgVM on x<0.25, 5 adapt cycles)",
+        "value": round(est_n / est_l, 1),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.cycles,
+        "warmup": 0,
+        "ms_per_step": round(est_l * 1e3 / max(1, args.cycles * len(problems)), 4),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64 assembly and solves; bf16 weights, f32 accumulation in the network",
+        "data": "synthetic (seeded cube and bar problems, random material and load)",
+        "config": {"workload": f"cube (neo-Hookean, 8^3 hex base, to {args.final_elems} tets) and bar (J2, "
+                               f"32x4x4 hex base, to {args.final_elems_bar} tets), one load step, AvgVM over "
+                               f"elements with centroid x<0.25, {args.cycles} adapt cycles; reference J from the "
+                               f"1:8 refinement of the final nested-adapted mesh",
+                   "final_elems": args.final_elems, "final_elems_bar": args.final_elems_bar,
+                   "cycles": args.cycles},
+        "roofline": None,
+        "cpu_baseline": None,
+        "detail": {"estimate_seconds_nested": round(est_n, 4), "estimate_seconds_learned": round(est_l, 5),
+                   "effectivity_pairs_same_mesh": [[round(a, 4), round(b, 4)] for a, b in same],
+                   "mean_abs_effectivity_gap": round(float(np.mean(gaps)), 4) if gaps else None,
+                   "max_abs_effectivity_gap": round(float(np.max(gaps)), 4) if gaps else None,
+                   "mean_rel_effectivity_gap": round(float(np.mean(rel)), 4) if rel else None,
+                   "final_qoi_error": final_err,
+                   "learned_fallbacks": fallbacks, "training": train_info, "device": name,
+                   "problems": report},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,9 +427,19 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="hexahedra per cube side")
     ap.add_argument("--order", type=int, default=2)
-    ap.add_argument("--workload", default="cube", choices=("cube", "bar", "batch"),
+    ap.add_argument("--workload", default="cube", choices=("cube", "bar", "batch", "learned"),
                     help="cube: P2 neo-Hookean cube (the headline); bar: P1 J2 bar, 5.3 M tets; "
-                         "batch: 64 meshes of uneven size in one graph launch")
+                         "batch: 64 meshes of uneven size in one graph launch; learned: adaptation with the "
+                         "learned indicator against plain nested DWR")
+    ap.add_argument("--model", default="", help="learned: trained network (.npz); default the packaged one")
+    ap.add_argument("--retrain", action="store_true", help="learned: build the dataset and train first")
+    ap.add_argument("--save-model", default="", help="learned: where a retrained network is written")
+    ap.add_argument("--train-runs", type=int, default=200)
+    ap.add_argument("--train-only", action="store_true", help="learned: stop after training")
+    ap.add_argument("--cycles", type=int, default=5)
+    ap.add_argument("--final-elems", type=int, default=2_000_000, help="learned: final size of the cube runs")
+    ap.add_argument("--final-elems-bar", type=int, default=200_000, help="learned: final size of the bar runs")
+    ap.add_argument("--eval-seed", type=int, default=2024)
     ap.add_argument("--blocks", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="hexahedra per side of the CPU baseline sample")
@@ -370,6 +478,8 @@ def main() -> int:
     log(f"device: {name}, {cus} CUs")
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "learned":
+        return bench_learned(args)
     wl = workload(args.workload, args.n, args.order)
     order, quad = wl["order"], wl["quad"]
     n = args.n

@@ -89,6 +89,12 @@ _sig("goal_mesh_box", _i32, _i32, _i32, _i32, _d, _d, _d, _i32, _d, C.c_uint64, 
 _sig("goal_mesh_graph", _i32, _i32, _i64, _i64, _p, _i64p, _p, _p)
 _sig("goal_mesh_refine", _i32, _i64, _p, _i64, _p, _p, _p, _p, _p, _p)
 _sig("goal_disc_create", _i32, _i32, _i64, _p, _i64, _p, _p, C.POINTER(_p))
+_sig("goal_size_field", _i32, _i64, _p, _i64, _p, _p, _i64, _i32, _p)
+_sig("goal_adapt_create", _i32, _i64, _p, _i64, _p, _p, _d, _i32, C.POINTER(_p))
+_sig("goal_adapt_destroy", _i32, _p)
+_sig("goal_adapt_sizes", _i32, _p, _p)
+_sig("goal_adapt_get", _i32, _p, _p, _p, _p, _p)
+_sig("goal_adapt_transfer", _i32, _p, _i32, _p, _p)
 _sig("goal_disc_create_box", _i32, _i32, _i32, _i32, _d, _d, _d, _i32, _d, C.c_uint64, C.POINTER(_p))
 _sig("goal_disc_destroy", _i32, _p)
 _sig("goal_disc_sizes", _i32, _p, _p)
@@ -124,6 +130,7 @@ _sig("goal_solinfo_norm_R", _i32, _p, _dp)
 _sig("goal_solinfo_dot_R", _i32, _p, _p, _dp)
 _sig("goal_linear_solve", _i32, _p, _p, _p, C.c_double, _i32, _i32, C.POINTER(_i32), _dp)
 _sig("goal_device_fill", _i32, _p, _i64, C.c_double)
+_sig("goal_device_scale", _i32, _p, _p, _i64, C.c_double)
 _sig("goal_mech_create", _i32, _p, _p, _i32, _p, _i32, _i32, C.POINTER(_p))
 _sig("goal_mech_destroy", _i32, _p)
 _sig("goal_mech_set_option", _i32, _p, C.c_char_p, _i64)
@@ -141,6 +148,14 @@ _sig("goal_batch_destroy", _i32, _p)
 _sig("goal_batch_tune", _i32, _p, _i32, _p, _p)
 _sig("goal_batch_capture", _i32, _p)
 _sig("goal_batch_run", _i32, _p)
+_sig("goal_learn_features", _i32, _p, _d, _p)
+_sig("goal_mlp_create", _i32, C.POINTER(_p))
+_sig("goal_mlp_destroy", _i32, _p)
+_sig("goal_mlp_set", _i32, _p, _p, _p)
+_sig("goal_mlp_get", _i32, _p, _p, _p, _p)
+_sig("goal_mlp_forward", _i32, _p, _p, _i64, _p, _i64p)
+_sig("goal_mlp_gradient", _i32, _p, _p, _p, _p, _i64, C.POINTER(C.c_float), _p)
+_sig("goal_mlp_train", _i32, _p, _p, _p, _p, _i64, _i32, C.c_float, C.POINTER(C.c_float))
 _sig("goal_nested_create", _i32, _p, C.POINTER(_p))
 _sig("goal_nested_destroy", _i32, _p)
 _sig("goal_nested_disc", _p, _p)
@@ -238,6 +253,52 @@ def mesh_refine(coords, conn):
     return fc, fconn, parent, ep
 
 
+def size_field(coords, conn, eta, target: int, p_order: int = 1) -> np.ndarray:
+    """Per-vertex target size from per-element indicators (linear mesh)."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    out = np.empty(coords.shape[0])
+    check(lib.goal_size_field(coords.shape[0], _ptr(coords), conn.shape[0], _ptr(conn), _ptr(eta), int(target),
+                              p_order, _ptr(out)))
+    return out
+
+
+class Adaptation:
+    """Refinement of a linear mesh toward a vertex size field (host)."""
+
+    def __init__(self, coords, conn, vtx_size, split_ratio: float = 1.5, max_passes: int = 8):
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        vtx_size = np.ascontiguousarray(vtx_size, dtype=np.float64)
+        self.h = _p()
+        check(lib.goal_adapt_create(coords.shape[0], _ptr(coords), conn.shape[0], _ptr(conn), _ptr(vtx_size),
+                                    float(split_ratio), int(max_passes), C.byref(self.h)))
+        sz = np.zeros(5, dtype=np.int64)
+        check(lib.goal_adapt_sizes(self.h, _ptr(sz)))
+        self.num_nodes, self.num_elems, self.num_new, self.splits, self.passes = (int(v) for v in sz)
+        self.num_old = self.num_nodes - self.num_new
+        self.coords = np.empty((self.num_nodes, 3))
+        self.conn = np.empty((self.num_elems, 4), dtype=np.int32)
+        self.parent = np.empty(self.num_elems, dtype=np.int32)
+        self.new_vertex_parents = np.empty((self.num_new, 2), dtype=np.int32)
+        check(lib.goal_adapt_get(self.h, _ptr(self.coords), _ptr(self.conn), _ptr(self.parent),
+                                 _ptr(self.new_vertex_parents)))
+
+    def transfer(self, field) -> np.ndarray:
+        """Nodal field [num_old, k] -> [num_nodes, k] (new vertices: edge means)."""
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        k = 1 if f.ndim == 1 else f.shape[1]
+        out = np.empty((self.num_nodes, k) if f.ndim > 1 else self.num_nodes)
+        check(lib.goal_adapt_transfer(self.h, k, _ptr(f), _ptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.goal_adapt_destroy(self.h)
+            self.h = _p()
+
+
 class DeviceArray:
     """A device buffer of float64 owned by this object."""
 
@@ -265,6 +326,102 @@ class DeviceArray:
         if getattr(self, "ptr", None) and self.ptr.value:
             lib.goal_device_free(self.ptr)
             self.ptr = _p()
+
+
+class DeviceBuffer:
+    """A device buffer of n items of a numpy dtype, owned by this object."""
+
+    def __init__(self, n: int, dtype=np.float32):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.ptr = _p()
+        check(lib.goal_device_alloc(max(1, self.n) * self.dtype.itemsize, C.byref(self.ptr)))
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, dtype=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size > self.n:
+            raise ValueError("DeviceBuffer.upload: array larger than the buffer")
+        check(lib.goal_memcpy_h2d(self.ptr, _ptr(a), a.nbytes))
+
+    def numpy(self, n: int | None = None) -> np.ndarray:
+        out = np.empty(self.n if n is None else int(n), dtype=self.dtype)
+        check(lib.goal_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib.goal_device_free(self.ptr)
+            self.ptr = _p()
+
+
+MLP_IN, MLP_HIDDEN, MLP_PARAMS, MLP_NORMS = 12, 64, 5057, 51
+
+
+def learn_features(mech: "Mechanics", qoi_x: float, out: DeviceBuffer | None = None) -> DeviceBuffer:
+    """Per-element features [num_elems][12] (FP32, device) of mech's disc."""
+    n = mech.disc.num_elems * MLP_IN
+    if out is None or out.n < n:
+        out = DeviceBuffer(n, np.float32)
+    check(lib.goal_learn_features(mech.h, float(qoi_x), out.ptr))
+    return out
+
+
+class MLP:
+    """12 -> 64 -> 64 -> 1 ReLU network on the device (BF16 weights)."""
+
+    def __init__(self):
+        self.h = _p()
+        check(lib.goal_mlp_create(C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.goal_mlp_destroy(self.h)
+            self.h = _p()
+
+    def set(self, params=None, norms=None) -> None:
+        pa = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        na = None if norms is None else np.ascontiguousarray(norms, dtype=np.float32)
+        if pa is not None and pa.size != MLP_PARAMS:
+            raise ValueError("MLP.set: params must have %d entries" % MLP_PARAMS)
+        if na is not None and na.size != MLP_NORMS:
+            raise ValueError("MLP.set: norms must have %d entries" % MLP_NORMS)
+        check(lib.goal_mlp_set(self.h, _ptr(pa), _ptr(na)))
+
+    def get(self):
+        """(FP32 master parameters, BF16 weights as uint16, normalization)."""
+        pa = np.empty(MLP_PARAMS, dtype=np.float32)
+        wq = np.empty(MLP_PARAMS, dtype=np.uint16)
+        na = np.empty(MLP_NORMS, dtype=np.float32)
+        check(lib.goal_mlp_get(self.h, _ptr(pa), _ptr(wq), _ptr(na)))
+        return pa, wq, na
+
+    def forward(self, x: DeviceBuffer, n: int, y: DeviceBuffer) -> int:
+        """y[:n] = predictions for rows x[:n]; returns rows outside the training box."""
+        if x.n < n * MLP_IN or y.n < n:
+            raise ValueError("MLP.forward: buffers too small")
+        out = C.c_int64(0)
+        check(lib.goal_mlp_forward(self.h, x.ptr, int(n), y.ptr, C.byref(out)))
+        return out.value
+
+    def gradient(self, x: DeviceBuffer, y: DeviceBuffer, idx: DeviceBuffer, n: int):
+        loss = C.c_float(0.0)
+        grad = np.empty(MLP_PARAMS, dtype=np.float32)
+        check(lib.goal_mlp_gradient(self.h, x.ptr, y.ptr, idx.ptr, int(n), C.byref(loss), _ptr(grad)))
+        return loss.value, grad
+
+    def train(self, x: DeviceBuffer, y: DeviceBuffer, idx: DeviceBuffer, batch: int, steps: int, lr: float) -> float:
+        if idx.n < batch * steps:
+            raise ValueError("MLP.train: index buffer shorter than batch * steps")
+        loss = C.c_float(0.0)
+        check(lib.goal_mlp_train(self.h, x.ptr, y.ptr, idx.ptr, int(batch), int(steps), float(lr), C.byref(loss)))
+        return loss.value
 
 
 class Disc:
@@ -439,12 +596,15 @@ class SolInfo:
         check(lib.goal_solinfo_dev(self.h, C.byref(r), C.byref(m), C.byref(v)))
         return r.value, m.value, v.value
 
-    def solve(self, b: "DeviceArray", x: "DeviceArray", rtol=1e-10, max_iters=2000, restart=100):
+    def solve(self, b, x, rtol=1e-10, max_iters=2000, restart=100):
         """dRdu x = b on the device (GMRES, node block-Jacobi); x holds the
-        initial guess.  Returns (inner iterations, relative residual ||b - Ax|| / ||b - Ax0||)."""
+        initial guess.  b, x: DeviceArray or raw device pointers.  Returns
+        (inner iterations, relative residual ||b - Ax|| / ||b - Ax0||)."""
         it = C.c_int(0)
         rr = C.c_double(0.0)
-        check(lib.goal_linear_solve(self.h, b.ptr, x.ptr, rtol, max_iters, restart, C.byref(it), C.byref(rr)))
+        bp = b.ptr if isinstance(b, DeviceArray) else b
+        xp = x.ptr if isinstance(x, DeviceArray) else x
+        check(lib.goal_linear_solve(self.h, bp, xp, rtol, max_iters, restart, C.byref(it), C.byref(rr)))
         return it.value, rr.value
 
     def norm_R(self) -> float:

@@ -44,12 +44,14 @@ HIP_FLAGS = [
 
 SOURCES = [
     "goal_mesh.cpp",
+    "goal_adapt.cpp",
     "goal_expr.cpp",
     "goal_capi.cpp",
     "goal_assembly.hip",
     "goal_util.hip",
     "goal_dwr.hip",
     "goal_solve.hip",
+    "goal_learn.hip",
 ]
 
 

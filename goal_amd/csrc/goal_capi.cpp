@@ -199,6 +199,7 @@ struct goal_mech_s {
   DevBuf<int32_t> status;
   DevBuf<double> elem_value;
   DevBuf<double> emr, em;  // element residuals / matrices between the two passes
+  DevBuf<double> feat_vol, feat_vm;  // learned-indicator feature scratch
   bool timing = false, timed = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   ~goal_mech_s() {
@@ -217,6 +218,16 @@ struct goal_batch_s {
   ~goal_batch_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+struct goal_mlp_s {
+  DevBuf<float> params, norms, adam_m, adam_v, grad, partial, partial_loss, loss;
+  DevBuf<uint16_t> wq;
+  DevBuf<int> ood;
+  int step = 0;
+  goal::MlpArgs args() {
+    return goal::MlpArgs{params.p, wq.p, norms.p, adam_m.p, adam_v.p};
   }
 };
 
@@ -457,6 +468,13 @@ int goal_device_copy(double* dst, const double* src, int64_t n) {
     return GOAL_OK;
   });
 }
+int goal_device_scale(double* y, const double* x, int64_t n, double a) {
+  return guard([&]() -> int {
+    require(y && x && n >= 0, "goal_device_scale: bad arguments");
+    HIPCHK(goal::launch_scale(n, a, x, y, stream()));
+    return GOAL_OK;
+  });
+}
 int goal_device_fill(double* p, int64_t n, double value) {
   return guard([&]() -> int {
     require(p && n >= 0, "goal_device_fill: bad arguments");
@@ -536,6 +554,103 @@ int goal_mesh_refine(int64_t num_nodes, const double* coords, int64_t num_elems,
     if (fine_conn) std::memcpy(fine_conn, r.fine.conn.data(), sizeof(int32_t) * r.fine.conn.size());
     if (parent) std::memcpy(parent, r.parent.data(), sizeof(int32_t) * r.parent.size());
     if (edge_parents) std::memcpy(edge_parents, r.edge_parents.data(), sizeof(int32_t) * r.edge_parents.size());
+    return GOAL_OK;
+  });
+}
+
+// ----------------------------------------------------------------- adapt
+struct goal_adapt_s {
+  goal::Adaptation a;
+};
+
+namespace {
+goal::HostMesh linear_mesh(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn) {
+  goal::HostMesh m;
+  m.order = 1;
+  m.nodes_per_elem = 4;
+  m.num_nodes = num_nodes;
+  m.num_elems = num_elems;
+  m.coords.assign(coords, coords + num_nodes * 3);
+  m.conn.assign(conn, conn + num_elems * 4);
+  goal::finalize_mesh(m);
+  return m;
+}
+}  // namespace
+
+int goal_size_field(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
+                    const double* eta, int64_t target, int p_order, double* vtx_size) {
+  return guard([&]() -> int {
+    require(coords && conn && eta && vtx_size && num_nodes > 0 && num_elems > 0 && p_order >= 1,
+            "goal_size_field: bad arguments");
+    const goal::HostMesh m = linear_mesh(num_nodes, coords, num_elems, conn);
+    const std::vector<double> s = goal::iso_target_size(m, eta, target, p_order);
+    std::memcpy(vtx_size, s.data(), sizeof(double) * s.size());
+    return GOAL_OK;
+  });
+}
+
+int goal_adapt_create(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
+                      const double* vtx_size, double split_ratio, int max_passes, goal_adapt* out) {
+  return guard([&]() -> int {
+    require(out && coords && conn && vtx_size && num_nodes > 0 && num_elems > 0 && max_passes >= 0,
+            "goal_adapt_create: bad arguments");
+    const goal::HostMesh m = linear_mesh(num_nodes, coords, num_elems, conn);
+    std::vector<double> size(vtx_size, vtx_size + num_nodes);
+    for (double h : size) require(h > 0.0 && std::isfinite(h), "goal_adapt_create: sizes must be positive");
+    auto* a = new goal_adapt_s;
+    try {
+      a->a = goal::adapt_refine(m, size, split_ratio, max_passes);
+      require((int64_t)a->a.mesh.num_nodes <= (int64_t)INT32_MAX / 4, "goal_adapt_create: mesh too large");
+    } catch (...) {
+      delete a;
+      throw;
+    }
+    *out = a;
+    return GOAL_OK;
+  });
+}
+
+int goal_adapt_destroy(goal_adapt a) {
+  delete a;
+  return GOAL_OK;
+}
+
+int goal_adapt_sizes(goal_adapt a, int64_t* sizes) {
+  return guard([&]() -> int {
+    require(a && sizes, "goal_adapt_sizes: bad arguments");
+    sizes[0] = a->a.mesh.num_nodes;
+    sizes[1] = a->a.mesh.num_elems;
+    sizes[2] = (int64_t)a->a.new_vertex_parents.size() / 2;
+    sizes[3] = a->a.splits;
+    sizes[4] = a->a.passes;
+    return GOAL_OK;
+  });
+}
+
+int goal_adapt_get(goal_adapt a, double* coords, int32_t* conn, int32_t* parent, int32_t* new_vertex_parents) {
+  return guard([&]() -> int {
+    require(a != nullptr, "goal_adapt_get: bad arguments");
+    const goal::Adaptation& r = a->a;
+    if (coords) std::memcpy(coords, r.mesh.coords.data(), sizeof(double) * r.mesh.coords.size());
+    if (conn) std::memcpy(conn, r.mesh.conn.data(), sizeof(int32_t) * r.mesh.conn.size());
+    if (parent) std::memcpy(parent, r.parent.data(), sizeof(int32_t) * r.parent.size());
+    if (new_vertex_parents)
+      std::memcpy(new_vertex_parents, r.new_vertex_parents.data(), sizeof(int32_t) * r.new_vertex_parents.size());
+    return GOAL_OK;
+  });
+}
+
+int goal_adapt_transfer(goal_adapt a, int ncomp, const double* old_field, double* new_field) {
+  return guard([&]() -> int {
+    require(a && old_field && new_field && ncomp > 0, "goal_adapt_transfer: bad arguments");
+    const goal::Adaptation& r = a->a;
+    const int64_t nold = r.num_old_vertices, nnew = (int64_t)r.new_vertex_parents.size() / 2;
+    std::memcpy(new_field, old_field, sizeof(double) * nold * ncomp);
+    for (int64_t k = 0; k < nnew; ++k) {
+      const int32_t p0 = r.new_vertex_parents[2 * k], p1 = r.new_vertex_parents[2 * k + 1];
+      for (int c = 0; c < ncomp; ++c)
+        new_field[(nold + k) * ncomp + c] = 0.5 * (new_field[(int64_t)p0 * ncomp + c] + new_field[(int64_t)p1 * ncomp + c]);
+    }
     return GOAL_OK;
   });
 }
@@ -1334,6 +1449,181 @@ int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* va
     if (elem_values) m->elem_value.download(elem_values, d->mesh.num_elems);
     int st = finish_assembly(m);
     return st;
+  });
+}
+
+// ---------------------------------------------------- learned indicator
+int goal_learn_features(goal_mech m, double qoi_x, float* feat_dev) {
+  return guard([&]() -> int {
+    require(m && feat_dev, "goal_learn_features: bad arguments");
+    goal_disc d = m->d;
+    const int64_t ne = d->mesh.num_elems, nn = d->mesh.num_nodes;
+    // per-element integral of the von Mises stress (the AvgVM integrand over all elements)
+    goal::QoiArgs q{};
+    q.num_elems = ne;
+    q.conn = d->conn.p;
+    q.coords = d->coords.p;
+    q.x = d->x.p;
+    q.elem_set = d->elem_set.p;
+    q.mats = m->mats.p;
+    q.Fp_old = m->st->Fp_old.p;
+    q.eqps_old = m->st->eqps_old.p;
+    q.model = m->model;
+    q.kind = GOAL_QOI_AVG_VM;
+    q.qoi_set = -1;
+    q.elem_value = m->elem_value.p;
+    q.dMdu = nullptr;
+    q.status = m->status.p;
+    HIPCHK(goal::launch_qoi(d->mesh.order, m->quad_order, q, stream()));
+    require(d->mesh.num_elems > 0, "goal_learn_features: empty mesh");
+    if (m->feat_vol.n < 10 * ne) m->feat_vol.alloc(10 * ne);
+    if (m->feat_vm.n < 10 * nn) m->feat_vm.alloc(10 * nn);
+    goal::FeatureArgs f{};
+    f.num_elems = ne;
+    f.num_nodes = nn;
+    f.npe = d->mesh.nodes_per_elem;
+    f.conn = d->conn.p;
+    f.coords = d->coords.p;
+    f.x = d->x.p;
+    f.elem_set = d->elem_set.p;
+    f.mats = m->mats.p;
+    f.elem_vm = m->elem_value.p;
+    f.eqps = m->model == goal::MODEL_J2 ? m->st->eqps.p : nullptr;
+    f.nq = m->st->nq;
+    f.inc_ptr = d->inc_ptr.p;
+    f.inc = d->inc.p;
+    f.qoi_x = qoi_x;
+    f.elem_scratch = m->feat_vol.p;
+    f.node_scratch = m->feat_vm.p;
+    f.feat = feat_dev;
+    HIPCHK(goal::launch_features(f, stream()));
+    return finish_assembly(m);
+  });
+}
+
+int goal_mlp_create(goal_mlp* out) {
+  return guard([&]() -> int {
+    require(out != nullptr, "goal_mlp_create: bad arguments");
+    ctx();
+    auto* m = new goal_mlp_s;
+    try {
+      m->params.alloc(goal::kMlpParams);
+      m->wq.alloc(goal::kMlpParams);
+      m->norms.alloc(goal::kMlpNorms);
+      m->adam_m.alloc(goal::kMlpParams);
+      m->adam_v.alloc(goal::kMlpParams);
+      m->grad.alloc(goal::kMlpParams);
+      m->loss.alloc(1);
+      m->ood.alloc(1);
+      std::vector<float> zero(goal::kMlpParams, 0.0f), norms(goal::kMlpNorms, 0.0f);
+      for (int k = 0; k < goal::kMlpIn; ++k) {
+        norms[goal::kMlpIn + k] = 1.0f;
+        norms[goal::kMlpOffLo + k] = -INFINITY;
+        norms[goal::kMlpOffHi + k] = INFINITY;
+      }
+      norms[2 * goal::kMlpIn + 1] = 1.0f;
+      HIPCHK(hipMemcpyAsync(m->params.p, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice, stream()));
+      HIPCHK(hipMemcpyAsync(m->adam_m.p, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice, stream()));
+      HIPCHK(hipMemcpyAsync(m->adam_v.p, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice, stream()));
+      HIPCHK(hipMemcpyAsync(m->norms.p, norms.data(), 4 * norms.size(), hipMemcpyHostToDevice, stream()));
+      HIPCHK(goal::launch_mlp_round(m->args(), stream()));
+      sync();
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+    return GOAL_OK;
+  });
+}
+
+int goal_mlp_destroy(goal_mlp m) {
+  delete m;
+  return GOAL_OK;
+}
+
+int goal_mlp_set(goal_mlp m, const float* params, const float* norms) {
+  return guard([&]() -> int {
+    require(m != nullptr, "goal_mlp_set: bad arguments");
+    if (params) {
+      for (int p = 0; p < goal::kMlpParams; ++p) require(std::isfinite(params[p]), "goal_mlp_set: non-finite parameter");
+      std::vector<float> zero(goal::kMlpParams, 0.0f);
+      HIPCHK(hipMemcpyAsync(m->params.p, params, 4 * goal::kMlpParams, hipMemcpyHostToDevice, stream()));
+      HIPCHK(hipMemcpyAsync(m->adam_m.p, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice, stream()));
+      HIPCHK(hipMemcpyAsync(m->adam_v.p, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice, stream()));
+      HIPCHK(goal::launch_mlp_round(m->args(), stream()));
+      m->step = 0;
+    }
+    if (norms) {
+      for (int k = 0; k < goal::kMlpIn; ++k) require(norms[goal::kMlpIn + k] > 0.0f, "goal_mlp_set: input scale must be positive");
+      require(norms[2 * goal::kMlpIn + 1] > 0.0f, "goal_mlp_set: output scale must be positive");
+      HIPCHK(hipMemcpyAsync(m->norms.p, norms, 4 * goal::kMlpNorms, hipMemcpyHostToDevice, stream()));
+    }
+    sync();
+    return GOAL_OK;
+  });
+}
+
+int goal_mlp_get(goal_mlp m, float* params, uint16_t* weights_bf16, float* norms) {
+  return guard([&]() -> int {
+    require(m != nullptr, "goal_mlp_get: bad arguments");
+    if (params) m->params.download(params, goal::kMlpParams);
+    if (weights_bf16) m->wq.download(weights_bf16, goal::kMlpParams);
+    if (norms) m->norms.download(norms, goal::kMlpNorms);
+    return GOAL_OK;
+  });
+}
+
+int goal_mlp_forward(goal_mlp m, const float* x_dev, int64_t n, float* y_dev, int64_t* num_outside) {
+  return guard([&]() -> int {
+    require(m && n >= 0 && (n == 0 || (x_dev && y_dev)), "goal_mlp_forward: bad arguments");
+    HIPCHK(hipMemsetAsync(m->ood.p, 0, sizeof(int), stream()));
+    HIPCHK(goal::launch_mlp_forward(m->args(), x_dev, n, y_dev, m->ood.p, stream()));
+    if (num_outside) {
+      int c = 0;
+      HIPCHK(hipMemcpyAsync(&c, m->ood.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+      sync();
+      *num_outside = c;
+    }
+    return GOAL_OK;
+  });
+}
+
+namespace {
+void mlp_workspace(goal_mlp m, int64_t batch) {
+  const int nb = goal::mlp_grad_blocks(batch);
+  if (m->partial.n < (int64_t)nb * goal::kMlpParams) m->partial.alloc((int64_t)nb * goal::kMlpParams);
+  if (m->partial_loss.n < nb) m->partial_loss.alloc(nb);
+}
+}  // namespace
+
+int goal_mlp_gradient(goal_mlp m, const float* x_dev, const float* y_dev, const int32_t* idx_dev, int64_t n,
+                      float* loss, float* grad) {
+  return guard([&]() -> int {
+    require(m && x_dev && y_dev && idx_dev && n > 0, "goal_mlp_gradient: bad arguments");
+    mlp_workspace(m, n);
+    HIPCHK(goal::launch_mlp_grad(m->args(), x_dev, y_dev, idx_dev, n, m->partial.p, m->partial_loss.p, m->grad.p,
+                                 m->loss.p, stream()));
+    if (grad) m->grad.download(grad, goal::kMlpParams);
+    if (loss) m->loss.download(loss, 1);
+    sync();
+    return GOAL_OK;
+  });
+}
+
+int goal_mlp_train(goal_mlp m, const float* x_dev, const float* y_dev, const int32_t* idx_dev, int64_t batch,
+                   int steps, float lr, float* last_loss) {
+  return guard([&]() -> int {
+    require(m && x_dev && y_dev && idx_dev && batch > 0 && steps >= 0 && lr > 0.0f, "goal_mlp_train: bad arguments");
+    mlp_workspace(m, batch);
+    for (int k = 0; k < steps; ++k) {
+      HIPCHK(goal::launch_mlp_grad(m->args(), x_dev, y_dev, idx_dev + (int64_t)k * batch, batch, m->partial.p,
+                                   m->partial_loss.p, m->grad.p, m->loss.p, stream()));
+      HIPCHK(goal::launch_mlp_adam(m->args(), m->grad.p, ++m->step, lr, stream()));
+    }
+    if (last_loss) m->loss.download(last_loss, 1);
+    sync();
+    return GOAL_OK;
   });
 }
 

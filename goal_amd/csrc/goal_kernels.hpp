@@ -148,4 +148,61 @@ hipError_t launch_set_error(const double* eta_child, int64_t num_base, double* e
 hipError_t launch_transfer_state(const double* base, double* fine, int ncomp, int64_t num_fine, int nq,
                                  int64_t base_pts, hipStream_t s);
 
+
+// Learned indicator (goal_learn.hip).  Parameter vector layout (FP32 master
+// and its BF16 copy share it): W1[64][12], b1[64], W2[64][64], b2[64], W3[64], b3.
+constexpr int kMlpIn = 12, kMlpHidden = 64;
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpHidden * kMlpIn;
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpHidden;
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHidden * kMlpHidden;
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHidden;
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHidden;
+constexpr int kMlpParams = kMlpOffB3 + 1;
+// Normalization vector: input shift mu[12], input scale 1/s[12], output
+// shift and scale, the training feature box lo[12], hi[12], and c, the weight
+// of raw feature 0 added to the output:
+//   y = net((x - mu) * is) * s_y + mu_y + c * x[0].
+constexpr int kMlpOffLo = 2 * kMlpIn + 2;
+constexpr int kMlpOffHi = kMlpOffLo + kMlpIn;
+constexpr int kMlpOffC = kMlpOffHi + kMlpIn;
+constexpr int kMlpNorms = kMlpOffC + 1;
+
+struct MlpArgs {
+  float* params;       // [kMlpParams] FP32 master
+  uint16_t* wq;        // [kMlpParams] BF16 copy (weight entries are the ones read)
+  const float* norms;  // [kMlpNorms]
+  float* adam_m;       // [kMlpParams]
+  float* adam_v;       // [kMlpParams]
+};
+
+struct FeatureArgs {
+  int64_t num_elems, num_nodes;
+  int npe;
+  const int32_t* conn;
+  const double* coords;
+  const double* x;
+  const int32_t* elem_set;
+  const MatParams* mats;
+  const double* elem_vm;    // [ne] integral of the von Mises stress over each element
+  const double* eqps;       // [ne*nq] or null
+  int nq;
+  const int64_t* inc_ptr;   // node -> incidence records
+  const int32_t* inc;       // record -> e * npe + local node
+  double qoi_x;             // QoI subdomain {x < qoi_x}
+  double* elem_scratch;     // [ne][10] volume, grad u
+  double* node_scratch;     // [num_nodes][10] recovered vm, grad u
+  float* feat;              // [ne][kMlpIn]
+};
+
+hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
+hipError_t launch_mlp_forward(const MlpArgs& a, const float* x, int64_t n, float* y, int* ood, hipStream_t s);
+// Blocks of the gradient kernel for a batch of B rows (partial holds that many
+// kMlpParams rows and partial_loss that many floats).
+int mlp_grad_blocks(int64_t B);
+hipError_t launch_mlp_grad(const MlpArgs& a, const float* x, const float* y, const int32_t* idx, int64_t B,
+                           float* partial, float* partial_loss, float* grad, float* loss, hipStream_t s);
+hipError_t launch_mlp_adam(const MlpArgs& a, const float* grad, int step, float lr, hipStream_t s);
+hipError_t launch_mlp_round(const MlpArgs& a, hipStream_t s);
+
 }  // namespace goal

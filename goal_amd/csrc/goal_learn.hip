@@ -1,0 +1,436 @@
+// Learned element error indicator: per-element features of a primal state
+// and a 12 -> 64 -> 64 -> 1 ReLU network (BF16 weights, FP32 biases and
+// accumulation) with its forward pass, mean-squared-error gradient and Adam
+// update.  The network predicts log10(eta_E / E) where eta_E is the nested-
+// adjoint indicator of Nested::set_error (src/goal_nested.cpp:395-412); it
+// has no counterpart in the reference.
+//
+// Every reduction runs in a fixed order (per-block partial sums, then one
+// ordered pass), so a training run is bitwise reproducible.
+
+#include "goal_kernels.hpp"
+
+namespace goal {
+namespace {
+
+constexpr int kIn = kMlpIn, kHid = kMlpHidden;
+constexpr int oW1 = kMlpOffW1, oB1 = kMlpOffB1, oW2 = kMlpOffW2, oB2 = kMlpOffB2, oW3 = kMlpOffW3,
+              oB3 = kMlpOffB3, kP = kMlpParams;
+
+// ------------------------------------------------------------- features
+
+constexpr int kFeatThreads = 256;
+
+__device__ inline double tet_volume(const double* X, const int32_t* v) {
+  const double* a = X + 3 * v[0];
+  const double* b = X + 3 * v[1];
+  const double* c = X + 3 * v[2];
+  const double* d = X + 3 * v[3];
+  const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double e3[3] = {d[0] - a[0], d[1] - a[1], d[2] - a[2]};
+  return (e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+          e1[2] * (e2[0] * e3[1] - e2[1] * e3[0])) /
+         6.0;
+}
+
+// Reference-configuration gradient of the vertex-interpolated displacement:
+// H[i][j] = sum_k (U[k+1][i] - U[0][i]) Ji[j][k], Ji the inverse of the
+// matrix whose rows are the edges 01, 02, 03.
+__device__ inline void linear_grad(const double X[4][3], const double U[4][3], double H[9]) {
+  double J[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) J[r][c] = X[r + 1][c] - X[0][c];
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  double Ji[3][3];
+  Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+  Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+  Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += (U[k + 1][i] - U[0][i]) * Ji[j][k];
+      H[3 * i + j] = s;
+    }
+}
+
+__device__ inline void load_vertices(const FeatureArgs& a, int64_t e, double X[4][3], double U[4][3]) {
+  const int32_t* v = a.conn + e * a.npe;
+  for (int i = 0; i < 4; ++i)
+    for (int c = 0; c < 3; ++c) {
+      X[i][c] = a.coords[3 * (int64_t)v[i] + c];
+      U[i][c] = a.x[4 * (int64_t)v[i] + c];
+    }
+}
+
+// Per element: volume and grad u, into elem_scratch[e][0..9].
+__global__ __launch_bounds__(kFeatThreads) void elem_geometry_kernel(FeatureArgs a) {
+  const int64_t e = (int64_t)blockIdx.x * kFeatThreads + threadIdx.x;
+  if (e >= a.num_elems) return;
+  double X[4][3], U[4][3], H[9];
+  load_vertices(a, e, X, U);
+  linear_grad(X, U, H);
+  double* o = a.elem_scratch + 10 * e;
+  o[0] = tet_volume(a.coords, a.conn + e * a.npe);
+  for (int k = 0; k < 9; ++k) o[1 + k] = H[k];
+}
+
+// Volume-weighted nodal means of the element von Mises stress and of grad u:
+// node_scratch[n] = (sum int_e vm, sum |e| H_e) / sum |e| over the incident
+// elements in ascending order.
+__global__ __launch_bounds__(kFeatThreads) void recover_kernel(FeatureArgs a) {
+  const int64_t n = (int64_t)blockIdx.x * kFeatThreads + threadIdx.x;
+  if (n >= a.num_nodes) return;
+  double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double w = 0.0;
+  for (int64_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
+    const int64_t e = a.inc[k] / a.npe;
+    const double* g = a.elem_scratch + 10 * e;
+    s[0] += a.elem_vm[e];
+    for (int c = 0; c < 9; ++c) s[1 + c] += g[0] * g[1 + c];
+    w += g[0];
+  }
+  double* o = a.node_scratch + 10 * n;
+  for (int c = 0; c < 10; ++c) o[c] = w > 0.0 ? s[c] / w : 0.0;
+}
+
+__device__ inline float flog10(double v) { return (float)log10(v); }
+
+// One element per thread.  Features (FP32):
+//  0 log10 h (RMS edge length)        6 log10(vm / E + 1e-12)
+//  1-3 centroid x, y, z               7 log10(|vm - vm*| / E + 1e-12)
+//  4 log10(|x_c - qoi_x| + h)         8 log10(|grad u|_F + 1e-12)
+//  5 1 if x_c < qoi_x else 0          9 log10(|grad u - (grad u)*|_F + 1e-12)
+//                                     10 nu
+//                                     11 log10(mean point eqps + 1e-9)
+// vm is the element mean von Mises stress; a starred value is the mean over
+// the element's four vertices of the recovered nodal value.  grad u uses the
+// vertex displacements (linear interpolation).
+__global__ __launch_bounds__(kFeatThreads) void features_kernel(FeatureArgs a) {
+  const int64_t e = (int64_t)blockIdx.x * kFeatThreads + threadIdx.x;
+  if (e >= a.num_elems) return;
+  const int32_t* v = a.conn + e * a.npe;
+  double X[4][3];
+  for (int i = 0; i < 4; ++i)
+    for (int c = 0; c < 3; ++c) X[i][c] = a.coords[3 * (int64_t)v[i] + c];
+  const double* ge = a.elem_scratch + 10 * e;
+  double star[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 4; ++i) {
+    const double* nv = a.node_scratch + 10 * (int64_t)v[i];
+    for (int c = 0; c < 10; ++c) star[c] += 0.25 * nv[c];
+  }
+  double h2 = 0.0, d2 = 0.0;
+  for (int k = 0; k < 9; ++k) {
+    h2 += ge[1 + k] * ge[1 + k];
+    d2 += (ge[1 + k] - star[1 + k]) * (ge[1 + k] - star[1 + k]);
+  }
+  double l2 = 0.0;
+  for (int k = 0; k < 6; ++k) {
+    const int p = k < 3 ? 0 : (k < 5 ? 1 : 2);
+    const int q = k < 3 ? k + 1 : (k < 5 ? k - 1 : 3);
+    for (int c = 0; c < 3; ++c) l2 += (X[p][c] - X[q][c]) * (X[p][c] - X[q][c]);
+  }
+  const double h = sqrt(l2 / 6.0);
+  double xc[3];
+  for (int c = 0; c < 3; ++c) xc[c] = 0.25 * (X[0][c] + X[1][c] + X[2][c] + X[3][c]);
+  const MatParams m = a.mats[a.elem_set[e]];
+  const double E = 9.0 * m.kappa * m.mu / (3.0 * m.kappa + m.mu);
+  const double nu = (3.0 * m.kappa - 2.0 * m.mu) / (2.0 * (3.0 * m.kappa + m.mu));
+  const double vm = a.elem_vm[e] / ge[0];
+  double eq = 0.0;
+  if (a.eqps) {
+    for (int q = 0; q < a.nq; ++q) eq += a.eqps[e * a.nq + q];
+    eq /= a.nq;
+  }
+  float* f = a.feat + e * kIn;
+  f[0] = flog10(h);
+  f[1] = (float)xc[0];
+  f[2] = (float)xc[1];
+  f[3] = (float)xc[2];
+  f[4] = flog10(fabs(xc[0] - a.qoi_x) + h);
+  f[5] = xc[0] < a.qoi_x ? 1.0f : 0.0f;
+  f[6] = flog10(fabs(vm) / E + 1e-12);
+  f[7] = flog10(fabs(vm - star[0]) / E + 1e-12);
+  f[8] = flog10(sqrt(h2) + 1e-12);
+  f[9] = flog10(sqrt(d2) + 1e-12);
+  f[10] = (float)nu;
+  f[11] = flog10(eq + 1e-9);
+}
+
+// ---------------------------------------------------------------- network
+
+__device__ inline float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Round to nearest, ties to even (finite inputs).
+__device__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Network parameters staged in LDS as FP32: weights from their BF16 copy,
+// biases from the FP32 master.
+struct SharedNet {
+  float w1[kHid * kIn], b1[kHid], w2[kHid * kHid], b2[kHid], w3[kHid], b3;
+  float mu[kIn], is[kIn], mu_y, s_y, c;
+};
+
+__device__ void stage_net(SharedNet& S, const MlpArgs& a) {
+  for (int i = threadIdx.x; i < kHid * kIn; i += blockDim.x) S.w1[i] = bf16_to_f32(a.wq[oW1 + i]);
+  for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) S.w2[i] = bf16_to_f32(a.wq[oW2 + i]);
+  for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+    S.w3[i] = bf16_to_f32(a.wq[oW3 + i]);
+    S.b1[i] = a.params[oB1 + i];
+    S.b2[i] = a.params[oB2 + i];
+  }
+  for (int i = threadIdx.x; i < kIn; i += blockDim.x) {
+    S.mu[i] = a.norms[i];
+    S.is[i] = a.norms[kIn + i];
+  }
+  if (threadIdx.x == 0) {
+    S.b3 = a.params[oB3];
+    S.mu_y = a.norms[2 * kIn];
+    S.s_y = a.norms[2 * kIn + 1];
+    S.c = a.norms[kMlpOffC];
+  }
+  __syncthreads();
+}
+
+constexpr int kFwdThreads = 256;
+
+// y[i] = s_y * net((x[i] - mu) * is) + mu_y + c * x[i][0].  Rows outside the
+// training feature box [lo, hi] are counted in *ood.
+__global__ __launch_bounds__(kFwdThreads) void mlp_forward_kernel(MlpArgs a, const float* x, int64_t n, float* y,
+                                                                  int* ood) {
+  __shared__ SharedNet S;
+  stage_net(S, a);
+  const int64_t i = (int64_t)blockIdx.x * kFwdThreads + threadIdx.x;
+  if (i >= n) return;
+  float xs[kIn];
+  bool out = false;
+  const float x0 = x[i * kIn];
+  for (int k = 0; k < kIn; ++k) {
+    const float v = x[i * kIn + k];
+    out |= !(v >= a.norms[kMlpOffLo + k] && v <= a.norms[kMlpOffHi + k]);
+    xs[k] = (v - S.mu[k]) * S.is[k];
+  }
+  float h1[kHid];
+#pragma unroll
+  for (int j = 0; j < kHid; ++j) {
+    float s = S.b1[j];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) s += S.w1[j * kIn + k] * xs[k];
+    h1[j] = fmaxf(s, 0.0f);
+  }
+  float o = S.b3;
+  for (int jj = 0; jj < kHid; ++jj) {
+    float s = S.b2[jj];
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) s += S.w2[jj * kHid + j] * h1[j];
+    o += S.w3[jj] * fmaxf(s, 0.0f);
+  }
+  y[i] = o * S.s_y + S.mu_y + S.c * x0;
+  if (out && ood) atomicAdd(ood, 1);
+}
+
+// Gradient of L = (1/B) sum_i (net(x_i) - t_i)^2, t_i = (y_i - mu_y - c x_i0)/s_y, over
+// rows idx[0..B).  A block takes tiles of kTile rows: wave 0 runs the forward
+// and back-propagation of its rows into LDS, then all threads accumulate the
+// parameter gradients of the tile into registers.  Each block writes its sums
+// to partial[block][kP] and its loss sum to partial_loss[block].
+constexpr int kTile = 64, kGradThreads = 256, kPad = kHid + 1;
+
+struct SharedTile {
+  float xs[kTile][kIn + 1];
+  float h1[kTile][kPad], h2[kTile][kPad], d2[kTile][kPad], d1[kTile][kPad];
+  float d3[kTile];
+};
+
+__global__ __launch_bounds__(kGradThreads) void mlp_grad_kernel(MlpArgs a, const float* x, const float* y,
+                                                                const int32_t* idx, int64_t B, float* partial,
+                                                                float* partial_loss) {
+  __shared__ SharedNet S;
+  __shared__ SharedTile T;
+  stage_net(S, a);
+  const int tid = threadIdx.x;
+  // parameter ownership: W2 entries tid + 256 r (r < 16), W1 entries tid + 256 r
+  // (r < 3), and for tid < 193 one of W3 / b1 / b2 / b3
+  float g2[16], g1[3], g3 = 0.0f;
+  for (int r = 0; r < 16; ++r) g2[r] = 0.0f;
+  for (int r = 0; r < 3; ++r) g1[r] = 0.0f;
+  float loss = 0.0f;
+  const float invB = 1.0f / (float)B;
+  const int64_t tiles = (B + kTile - 1) / kTile;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    if (tid < kTile) {
+      const int64_t row = tile * kTile + tid;
+      const bool live = row < B;
+      float xs[kIn];
+      float t = 0.0f;
+      if (live) {
+        const int64_t r = idx[row];
+        for (int k = 0; k < kIn; ++k) xs[k] = (x[r * kIn + k] - S.mu[k]) * S.is[k];
+        t = (y[r] - S.mu_y - S.c * x[r * kIn]) / S.s_y;
+      } else {
+        for (int k = 0; k < kIn; ++k) xs[k] = 0.0f;
+      }
+      for (int k = 0; k < kIn; ++k) T.xs[tid][k] = xs[k];
+      // Each thread works on its own padded LDS rows, which keeps register
+      // use low; the odd row stride makes the 64 rows conflict-free.
+      for (int j = 0; j < kHid; ++j) {
+        float s = S.b1[j];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) s += S.w1[j * kIn + k] * xs[k];
+        T.h1[tid][j] = fmaxf(s, 0.0f);
+      }
+      float o = S.b3;
+      for (int jj = 0; jj < kHid; ++jj) {
+        float s = S.b2[jj];
+        for (int j = 0; j < kHid; ++j) s += S.w2[jj * kHid + j] * T.h1[tid][j];
+        const float h = fmaxf(s, 0.0f);
+        T.h2[tid][jj] = h;
+        o += S.w3[jj] * h;
+      }
+      const float r = live ? o - t : 0.0f;
+      loss += r * r;
+      const float d3 = 2.0f * r * invB;
+      T.d3[tid] = d3;
+      for (int jj = 0; jj < kHid; ++jj) T.d2[tid][jj] = T.h2[tid][jj] > 0.0f ? d3 * S.w3[jj] : 0.0f;
+      for (int j = 0; j < kHid; ++j) {
+        float s = 0.0f;
+        for (int jj = 0; jj < kHid; ++jj) s += S.w2[jj * kHid + j] * T.d2[tid][jj];
+        T.d1[tid][j] = T.h1[tid][j] > 0.0f ? s : 0.0f;
+      }
+    }
+    __syncthreads();
+    // W2 entry tid + 256 r is (row tid / 64 + 4 r, column tid % 64)
+    {
+      const int j = tid % kHid, jj0 = tid / kHid;
+      int j1[3], k1[3];
+      for (int r = 0; r < 3; ++r) {
+        j1[r] = (tid + kGradThreads * r) / kIn;
+        k1[r] = (tid + kGradThreads * r) % kIn;
+      }
+      const int sel = tid / kHid, col = tid % kHid;  // W3 / b1 / b2 slot for tid < 192
+#pragma unroll 2
+      for (int q = 0; q < kTile; ++q) {
+        const float h = T.h1[q][j];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) g2[r] += T.d2[q][jj0 + 4 * r] * h;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) g1[r] += T.d1[q][j1[r]] * T.xs[q][k1[r]];
+        if (sel == 0) g3 += T.d3[q] * T.h2[q][col];
+        else if (sel == 1) g3 += T.d1[q][col];
+        else if (sel == 2) g3 += T.d2[q][col];
+        else if (tid == 3 * kHid) g3 += T.d3[q];
+      }
+    }
+    __syncthreads();
+  }
+  float* out = partial + (int64_t)blockIdx.x * kP;
+  for (int r = 0; r < 16; ++r) out[oW2 + tid + kGradThreads * r] = g2[r];
+  for (int r = 0; r < 3; ++r) out[oW1 + tid + kGradThreads * r] = g1[r];
+  if (tid < kHid) out[oW3 + tid] = g3;
+  else if (tid < 2 * kHid) out[oB1 + tid - kHid] = g3;
+  else if (tid < 3 * kHid) out[oB2 + tid - 2 * kHid] = g3;
+  else if (tid == 3 * kHid) out[oB3] = g3;
+  if (tid < kTile) {
+    // wave 0 holds the tile losses: fixed-order wave sum
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+    if (tid == 0) partial_loss[blockIdx.x] = loss;
+  }
+}
+
+// grad[p] = sum_b partial[b][p]; loss = sum_b partial_loss[b] / B.
+__global__ void mlp_grad_reduce_kernel(const float* partial, const float* partial_loss, int nb, int64_t B,
+                                       float* grad, float* loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kP) {
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * kP + p];
+    grad[p] = s;
+  }
+  if (p == 0) {
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += partial_loss[b];
+    *loss = s / (float)B;
+  }
+}
+
+// Adam (Kingma & Ba 2015, algorithm 1) on the FP32 master; weight entries
+// are re-rounded into the BF16 copy.
+__global__ void mlp_adam_kernel(MlpArgs a, const float* grad, int step, float lr, float beta1, float beta2,
+                                float eps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kP) return;
+  const float g = grad[p];
+  const float m = beta1 * a.adam_m[p] + (1.0f - beta1) * g;
+  const float v = beta2 * a.adam_v[p] + (1.0f - beta2) * g * g;
+  a.adam_m[p] = m;
+  a.adam_v[p] = v;
+  const float mh = m / (1.0f - powf(beta1, (float)step));
+  const float vh = v / (1.0f - powf(beta2, (float)step));
+  const float w = a.params[p] - lr * mh / (sqrtf(vh) + eps);
+  a.params[p] = w;
+  a.wq[p] = f32_to_bf16(w);
+}
+
+__global__ void mlp_round_kernel(MlpArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kP) a.wq[p] = f32_to_bf16(a.params[p]);
+}
+
+}  // namespace
+
+int mlp_grad_blocks(int64_t B) {
+  const int64_t tiles = (B + kTile - 1) / kTile;
+  return (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), 512);
+}
+
+hipError_t launch_features(const FeatureArgs& a, hipStream_t s) {
+  if (a.num_elems == 0) return hipSuccess;
+  const unsigned ge = (unsigned)((a.num_elems + kFeatThreads - 1) / kFeatThreads);
+  const unsigned gn = (unsigned)((a.num_nodes + kFeatThreads - 1) / kFeatThreads);
+  hipLaunchKernelGGL(elem_geometry_kernel, dim3(ge), dim3(kFeatThreads), 0, s, a);
+  hipLaunchKernelGGL(recover_kernel, dim3(gn), dim3(kFeatThreads), 0, s, a);
+  hipLaunchKernelGGL(features_kernel, dim3(ge), dim3(kFeatThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const MlpArgs& a, const float* x, int64_t n, float* y, int* ood, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned g = (unsigned)((n + kFwdThreads - 1) / kFwdThreads);
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3(g), dim3(kFwdThreads), 0, s, a, x, n, y, ood);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad(const MlpArgs& a, const float* x, const float* y, const int32_t* idx, int64_t B,
+                           float* partial, float* partial_loss, float* grad, float* loss, hipStream_t s) {
+  if (B <= 0) return hipErrorInvalidValue;
+  const int nb = mlp_grad_blocks(B);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(nb), dim3(kGradThreads), 0, s, a, x, y, idx, B, partial, partial_loss);
+  hipLaunchKernelGGL(mlp_grad_reduce_kernel, dim3((kP + 255) / 256), dim3(256), 0, s, partial, partial_loss, nb, B,
+                     grad, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(const MlpArgs& a, const float* grad, int step, float lr, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kP + 255) / 256), dim3(256), 0, s, a, grad, step, lr, 0.9f, 0.999f,
+                     1e-8f);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_round(const MlpArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_round_kernel, dim3((kP + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace goal

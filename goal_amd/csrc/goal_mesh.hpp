@@ -99,6 +99,26 @@ struct Refinement {
 };
 Refinement refine_uniform(const HostMesh& base);
 
+// Isotropic target size per vertex from per-element error indicators
+// (get_iso_target_size, src/goal_size_field.cpp:39-151).  Linear meshes.
+std::vector<double> iso_target_size(const HostMesh& m, const double* eta, int64_t target, int p_order);
+
+// Refinement of a linear mesh toward a vertex size field by longest-edge
+// bisection.  Old vertices keep their ids; new vertex num_old_vertices + k is
+// the midpoint of new_vertex_parents[2k], [2k+1] (either may itself be a new
+// vertex with a smaller k).  parent[e] = element of the input mesh that
+// contains element e.
+struct Adaptation {
+  HostMesh mesh;
+  std::vector<int32_t> parent;
+  std::vector<int32_t> new_vertex_parents;
+  int64_t num_old_vertices = 0;
+  int64_t splits = 0;
+  int passes = 0;
+};
+Adaptation adapt_refine(const HostMesh& m, const std::vector<double>& vtx_size, double split_ratio,
+                        int max_passes);
+
 // splitmix64 of (seed, key): the jitter source, reproducible on any host.
 uint64_t splitmix64(uint64_t x);
 

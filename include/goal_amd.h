@@ -49,6 +49,8 @@ int goal_sync(void);
 int goal_device_copy(double* dst_dev, const double* src_dev, int64_t n);
 /* p[i] = value for i < n (async; a fill kernel, so it may be graph-captured). */
 int goal_device_fill(double* p_dev, int64_t n, double value);
+/* y[i] = a * x[i] for i < n (async; y may be x). */
+int goal_device_scale(double* y_dev, const double* x_dev, int64_t n, double a);
 /* HIP events on the library stream: record into slot 0..255, elapsed a->b in ms. */
 int goal_event_record(int slot);
 int goal_event_elapsed(int slot_a, int slot_b, float* ms);
@@ -70,6 +72,30 @@ int goal_mesh_graph(int order, int64_t num_nodes, int64_t num_elems, const int32
 int goal_mesh_refine(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
                      int64_t* sizes, double* fine_coords, int32_t* fine_conn, int32_t* parent,
                      int32_t* edge_parents);
+
+/* ------------------------------------------------------------ adaptation */
+/* get_iso_target_size (src/goal_size_field.cpp:39-151) on a linear mesh:
+ * per-element indicators eta [num_elems] -> per-vertex target size
+ * vtx_size [num_nodes] for a target element count (p_order 1 as in the
+ * reference's call). */
+int goal_size_field(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
+                    const double* eta, int64_t target, int p_order, double* vtx_size);
+/* The mesh step of Solver::adapt (src/main_nested.cpp:108-126, ma::adapt),
+ * refinement only: edges longer than split_ratio x the mean target size of
+ * their vertices are bisected (longest-edge propagation keeps the mesh
+ * conforming) for up to max_passes passes.  Old vertices keep their ids; new
+ * vertex num_old + k is the midpoint of new_vertex_parents[2k], [2k+1]. */
+typedef struct goal_adapt_s* goal_adapt;
+int goal_adapt_create(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
+                      const double* vtx_size, double split_ratio, int max_passes, goal_adapt* out);
+int goal_adapt_destroy(goal_adapt a);
+/* sizes[0..4] = nodes, elements, new vertices, edge cuts, passes run */
+int goal_adapt_sizes(goal_adapt a, int64_t* sizes);
+/* Any output may be NULL.  parent[e] = input element that contains e. */
+int goal_adapt_get(goal_adapt a, double* coords, int32_t* conn, int32_t* parent, int32_t* new_vertex_parents);
+/* A nodal field with ncomp values per vertex carried to the adapted mesh:
+ * old vertices keep their values, new ones take the mean of their edge's. */
+int goal_adapt_transfer(goal_adapt a, int ncomp, const double* old_field, double* new_field);
 
 /* -------------------------------------------------------- discretization */
 typedef struct goal_disc_s* goal_disc;
@@ -244,6 +270,50 @@ int goal_batch_tune(goal_batch b, int reps, int* blocks, float* ms);
 int goal_batch_capture(goal_batch b);
 /* Launch the captured graph on the library stream (async). */
 int goal_batch_run(goal_batch b);
+
+/* ----------------------------------------------------- learned indicator */
+/* No counterpart in the reference: a 12 -> 64 -> 64 -> 1 ReLU network that
+ * predicts log10(eta_E / E) for the nested-adjoint indicator eta_E of
+ * Nested::set_error (src/goal_nested.cpp:395-412) from 12 per-element features
+ * of the primal state, standing in for the nested adjoint solve.  Weights are
+ * BF16, biases FP32, all accumulation FP32; training keeps an FP32 master
+ * copy and runs Adam on the device. */
+#define GOAL_MLP_IN 12
+#define GOAL_MLP_HIDDEN 64
+/* W1[64][12], b1[64], W2[64][64], b2[64], W3[64], b3 */
+#define GOAL_MLP_PARAMS 5057
+/* input shift[12], input scale[12] (x_hat = (x - shift) * scale), output
+ * shift and scale, training feature box lo[12], hi[12], then c: the
+ * prediction is y = net(x_hat) * scale + shift + c * x[0] */
+#define GOAL_MLP_NORMS 51
+/* Features [num_elems][12] FP32 on the device, for the current solution and
+ * states of m's disc: log10 h, centroid x y z, log10(|x_c - qoi_x| + h),
+ * [x_c < qoi_x], log10(vm/E), log10(|vm - vm*|/E), log10|grad u|,
+ * log10|grad u - (grad u)*|, nu, log10(eqps + 1e-9).  vm is the element mean
+ * von Mises stress (src/goal_von_mises.cpp:7-17); a starred value is the mean
+ * over the element's vertices of the volume-weighted nodal average. */
+int goal_learn_features(goal_mech m, double qoi_x, float* feat_dev);
+typedef struct goal_mlp_s* goal_mlp;
+/* Zero parameters, identity normalization, unbounded feature box. */
+int goal_mlp_create(goal_mlp* out);
+int goal_mlp_destroy(goal_mlp m);
+/* params [GOAL_MLP_PARAMS] FP32 (weights are rounded to BF16 for use; Adam
+ * state restarts), norms [GOAL_MLP_NORMS]; either may be NULL. */
+int goal_mlp_set(goal_mlp m, const float* params, const float* norms);
+/* FP32 master parameters, the BF16 weights in use, the normalization; any may be NULL. */
+int goal_mlp_get(goal_mlp m, float* params, uint16_t* weights_bf16, float* norms);
+/* y_dev[i] = prediction for row i of x_dev [n][12]; *num_outside (may be
+ * NULL) = rows with a feature outside the training box. */
+int goal_mlp_forward(goal_mlp m, const float* x_dev, int64_t n, float* y_dev, int64_t* num_outside);
+/* Mean squared error over rows idx_dev[0..n) of (x_dev, y_dev) in the
+ * normalized output space and its gradient (host [GOAL_MLP_PARAMS]) with
+ * respect to the parameters as used (BF16 weights). */
+int goal_mlp_gradient(goal_mlp m, const float* x_dev, const float* y_dev, const int32_t* idx_dev, int64_t n,
+                      float* loss, float* grad);
+/* steps Adam steps (beta 0.9, 0.999, eps 1e-8); step k uses rows
+ * idx_dev[k*batch .. (k+1)*batch).  *last_loss = the last step's loss. */
+int goal_mlp_train(goal_mlp m, const float* x_dev, const float* y_dev, const int32_t* idx_dev, int64_t batch,
+                   int steps, float lr, float* last_loss);
 
 /* ------------------------------------------------------------ nested DWR */
 typedef struct goal_nested_s* goal_nested;

@@ -661,4 +661,116 @@ class NestedAdjoint {
   double bound_ = 0.0, zR_ = 0.0;
 };
 
+// get_iso_target_size (src/goal_size_field.cpp:142-151) on a linear disc:
+// per-element indicators -> per-vertex target sizes for `target` elements.
+inline std::vector<double> get_iso_target_size(Disc* d, std::vector<double> const& eta, int64_t target) {
+  if (d->get_order() != 1) throw Error(GOAL_ERR_INVALID, "size field: linear meshes only");
+  const int64_t nn = d->get_num_nodes(), ne = d->get_num_elems();
+  if ((int64_t)eta.size() != ne) throw Error(GOAL_ERR_INVALID, "size field: one indicator per element");
+  std::vector<double> coords((size_t)(3 * nn)), size((size_t)nn);
+  std::vector<int32_t> conn((size_t)(4 * ne));
+  check(goal_disc_get_mesh(d->handle(), coords.data(), conn.data(), nullptr));
+  check(goal_size_field(nn, coords.data(), ne, conn.data(), eta.data(), target, 1, size.data()));
+  return size;
+}
+
+// The mesh step of Solver::adapt (src/main_nested.cpp:108-126) without
+// coarsening: a new disc refined toward `size` by longest-edge bisection.
+// Children keep their parent's element set; the solution is carried over
+// (new vertices take their edge's mean).  Node sets are not carried over.
+struct Adapted {
+  std::unique_ptr<Disc> disc;
+  std::vector<int32_t> parent;  // element -> element of the input disc
+  int64_t splits = 0;
+};
+inline Adapted adapt(Disc* d, std::vector<double> const& size, double split_ratio = 1.5, int max_passes = 8) {
+  const int64_t nn = d->get_num_nodes(), ne = d->get_num_elems();
+  std::vector<double> coords((size_t)(3 * nn));
+  std::vector<int32_t> conn((size_t)(4 * ne)), es((size_t)ne);
+  check(goal_disc_get_mesh(d->handle(), coords.data(), conn.data(), es.data()));
+  goal_adapt a = nullptr;
+  check(goal_adapt_create(nn, coords.data(), ne, conn.data(), size.data(), split_ratio, max_passes, &a));
+  Adapted out;
+  try {
+    int64_t sz[5];
+    check(goal_adapt_sizes(a, sz));
+    std::vector<double> c2((size_t)(3 * sz[0]));
+    std::vector<int32_t> t2((size_t)(4 * sz[1]));
+    out.parent.resize((size_t)sz[1]);
+    check(goal_adapt_get(a, c2.data(), t2.data(), out.parent.data(), nullptr));
+    std::vector<int32_t> es2((size_t)sz[1]);
+    for (int64_t e = 0; e < sz[1]; ++e) es2[e] = es[out.parent[e]];
+    out.disc.reset(new Disc(1, sz[0], c2.data(), sz[1], t2.data(), es2.data()));
+    std::vector<double> x2((size_t)(4 * sz[0]));
+    const std::vector<double> x = d->get_soln();
+    check(goal_adapt_transfer(a, 4, x.data(), x2.data()));
+    out.disc->set_soln(x2);
+    out.splits = sz[3];
+  } catch (...) {
+    goal_adapt_destroy(a);
+    throw;
+  }
+  goal_adapt_destroy(a);
+  return out;
+}
+
+// Learned stand-in for NestedAdjoint::run (no counterpart in the reference):
+// per-element indicators eta_E = E * 10^net(features) from a trained network,
+// or the nested adjoint's when more than ood_limit of the elements have
+// features outside the training box.
+class LearnedIndicator {
+ public:
+  LearnedIndicator(std::vector<float> const& params, std::vector<float> const& norms, double qoi_x,
+                   double ood_limit = 0.01)
+      : qoi_x_(qoi_x), ood_limit_(ood_limit) {
+    if (params.size() != GOAL_MLP_PARAMS || norms.size() != GOAL_MLP_NORMS)
+      throw Error(GOAL_ERR_INVALID, "learned indicator: parameter or normalization size");
+    check(goal_mlp_create(&mlp_));
+    check(goal_mlp_set(mlp_, params.data(), norms.data()));
+  }
+  ~LearnedIndicator() {
+    if (mlp_) goal_mlp_destroy(mlp_);
+  }
+  LearnedIndicator(const LearnedIndicator&) = delete;
+  LearnedIndicator& operator=(const LearnedIndicator&) = delete;
+
+  std::vector<double> run(Mechanics* mech, NestedAdjoint* fallback, double t_now, double t_old) {
+    Disc* d = mech->get_disc();
+    const int64_t ne = d->get_num_elems();
+    float *feat = nullptr, *y = nullptr;
+    check(goal_device_alloc(4 * ne * GOAL_MLP_IN, (void**)&feat));
+    if (goal_device_alloc(4 * ne, (void**)&y) != GOAL_OK) {
+      goal_device_free(feat);
+      throw Error(GOAL_ERR_ALLOC, "learned indicator: device allocation");
+    }
+    std::vector<float> yh((size_t)ne);
+    int64_t outside = 0;
+    int st = goal_learn_features(mech->handle(), qoi_x_, feat);
+    if (st == GOAL_OK) st = goal_mlp_forward(mlp_, feat, ne, y, &outside);
+    if (st == GOAL_OK) st = goal_memcpy_d2h(yh.data(), y, 4 * ne);
+    goal_device_free(feat);
+    goal_device_free(y);
+    check(st);
+    fell_back_ = outside > ood_limit_ * (double)ne;
+    if (fell_back_) {
+      if (!fallback) throw Error(GOAL_ERR_INVALID, "learned indicator: features out of range and no fallback");
+      return fallback->run(t_now, t_old);
+    }
+    std::vector<int32_t> es((size_t)ne);
+    std::vector<double> coords((size_t)(3 * d->get_num_nodes()));
+    std::vector<int32_t> conn((size_t)(ne * (d->get_order() == 1 ? 4 : 10)));
+    check(goal_disc_get_mesh(d->handle(), coords.data(), conn.data(), es.data()));
+    std::vector<double> eta((size_t)ne);
+    auto const& mats = mech->params().materials;
+    for (int64_t e = 0; e < ne; ++e) eta[e] = mats.at((size_t)es[e]).E * std::pow(10.0, (double)yh[e]);
+    return eta;
+  }
+  bool fell_back() const { return fell_back_; }
+
+ private:
+  goal_mlp mlp_ = nullptr;
+  double qoi_x_, ood_limit_;
+  bool fell_back_ = false;
+};
+
 }  // namespace goal

@@ -1,0 +1,198 @@
+"""Learned indicator on the GPU through the C ABI: the feature kernel, the
+network's forward pass, its gradient and the Adam step against the float64
+oracle (oracle/learned.py), bitwise reproducibility of training, and the
+adaptation pipeline that uses them.
+
+Tolerances: features are FP32 roundings of FP64 values (2e-6); the network
+accumulates 4,928 FP32 products per row (forward 1e-5 of the output scale,
+gradient 1e-4 normwise).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import goal_amd as g
+from goal_amd import learned as LP
+from oracle import learned as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(geometry, n):
+    if geometry == "cube":
+        return LP.Problem("cube", 1000.0, 0.25, load=(0.01, -0.005, 0.02), n=n)
+    return LP.Problem("bar", 200.0, 0.3, K=2.0, Y=0.25, load=(0.012, 0.01, -0.01), n=n)
+
+
+@pytest.mark.parametrize("geometry,n", [("cube", 3), ("bar", 2)])
+def test_features_match_oracle(geometry, n):
+    p = _problem(geometry, n)
+    m = LP.Model(p, *LP.base_mesh(p))
+    info = m.newton()
+    print(geometry, "newton", info["iters"], "linear", info["linear_iters"], flush=True)
+    assert info["converged"]
+    feat = LP.features(m).numpy(m.ne * 12).reshape(m.ne, 12)
+    _, ev = m.mech.qoi(LP.AVG_VM, None, elem_set=-1, elem_values=True)
+    x = m.disc.soln().reshape(-1, 4)
+    eq = m.states.get("eqps")[0] if p.model == 2 else None
+    ref = L.features(m.coords, m.conn, x, ev, p.E, p.nu, eq, LP.QOI_X)
+    assert np.allclose(feat, ref, rtol=2e-6, atol=2e-6)
+    if p.model == 2:
+        assert (eq > 0).any()  # the bar yields somewhere
+
+
+def _net(seed, n=5000):
+    rng = np.random.default_rng(seed)
+    params = LP.he_init(rng)
+    params[L.OFF_B1:L.OFF_W2] = rng.standard_normal(64) * 0.1
+    params[L.OFF_B2:L.OFF_W3] = rng.standard_normal(64) * 0.1
+    params[L.OFF_B3] = 0.2
+    norms = np.zeros(L.NNORMS, dtype=np.float32)
+    norms[:12] = rng.standard_normal(12) * 0.2
+    norms[12:24] = rng.uniform(0.5, 2.0, 12)
+    norms[24], norms[25] = -3.0, 1.5
+    norms[L.OFF_LO:L.OFF_HI] = -np.inf
+    norms[L.OFF_HI:L.OFF_C] = np.inf
+    norms[L.OFF_C] = 5.0
+    X = rng.standard_normal((n, 12)).astype(np.float32)
+    Y = (rng.standard_normal(n) * 2.0 - 3.0).astype(np.float32)
+    return params, norms, X, Y
+
+
+def test_mlp_forward_matches_oracle():
+    params, norms, X, _ = _net(0)
+    n = X.shape[0]
+    norms[L.OFF_LO + 3], norms[L.OFF_HI + 3] = -1.0, 1.5   # box one feature
+    mlp = g.MLP()
+    mlp.set(params, norms)
+    _, wq, nr = mlp.get()
+    assert np.array_equal(wq.astype(np.uint32) << 16, L.bf16(params).view(np.uint32))  # the BF16 copy
+    assert np.array_equal(nr, norms)
+    xd = g.DeviceBuffer.from_host(X.ravel(), np.float32)
+    yd = g.DeviceBuffer(n, np.float32)
+    outside = mlp.forward(xd, n, yd)
+    ref = L.mlp_forward(params, norms, X)
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(yd.numpy() - ref)) <= 1e-5 * scale
+    assert outside == int(np.count_nonzero((X[:, 3] < -1.0) | (X[:, 3] > 1.5)))
+
+
+def test_mlp_gradient_matches_oracle():
+    params, norms, X, Y = _net(1)
+    mlp = g.MLP()
+    mlp.set(params, norms)
+    rng = np.random.default_rng(7)
+    idx = rng.choice(X.shape[0], 1777, replace=False).astype(np.int32)   # not a multiple of the 64-row tile
+    xd = g.DeviceBuffer.from_host(X.ravel(), np.float32)
+    yd = g.DeviceBuffer.from_host(Y, np.float32)
+    loss, grad = mlp.gradient(xd, yd, g.DeviceBuffer.from_host(idx, np.int32), idx.size)
+    rl, rg = L.mlp_loss_grad(params, norms, X[idx], Y[idx])
+    assert abs(loss - rl) <= 1e-5 * rl
+    assert np.linalg.norm(grad - rg) <= 1e-4 * np.linalg.norm(rg)
+    for lo, hi in ((L.OFF_W1, L.OFF_B1), (L.OFF_W2, L.OFF_B2), (L.OFF_W3, L.OFF_B3 + 1)):
+        assert np.linalg.norm(grad[lo:hi] - rg[lo:hi]) <= 1e-4 * np.linalg.norm(rg[lo:hi])
+
+
+def test_adam_step_matches_oracle_and_training_is_reproducible():
+    params, norms, X, Y = _net(2)
+    n = X.shape[0]
+    xd = g.DeviceBuffer.from_host(X.ravel(), np.float32)
+    yd = g.DeviceBuffer.from_host(Y, np.float32)
+    rng = np.random.default_rng(3)
+    batch, steps, lr = 512, 20, 1e-3
+    idx_h = np.concatenate([rng.permutation(n)[:batch] for _ in range(steps)]).astype(np.int32)
+    idx = g.DeviceBuffer.from_host(idx_h, np.int32)
+    mlp = g.MLP()
+    mlp.set(params, norms)
+    _, g0 = mlp.gradient(xd, yd, idx, batch)
+    mlp.train(xd, yd, idx, batch, 1, lr)
+    p1, wq1, _ = mlp.get()
+    ref, _, _ = L.adam(params.astype(np.float64), np.zeros(L.NPARAMS), np.zeros(L.NPARAMS), g0.astype(np.float64),
+                       1, lr)
+    assert np.max(np.abs(p1 - ref)) <= 1e-6 * lr + 1e-7 * np.abs(ref).max()
+    assert np.array_equal(wq1.astype(np.uint32) << 16, L.bf16(p1).view(np.uint32))
+    # whole runs are bitwise reproducible (fixed-order reductions)
+    runs = []
+    for _ in range(2):
+        m = g.MLP()
+        m.set(params, norms)
+        last = m.train(xd, yd, idx, batch, steps, lr)
+        runs.append((m.get()[0], last))
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+
+
+def test_training_fits_a_known_function():
+    """A smooth function of the features is learned: the held-out error drops
+    far below the spread of the target."""
+    rng = np.random.default_rng(5)
+    n = 40000
+    X = rng.standard_normal((n, 12)).astype(np.float32)
+    Y = (np.sin(X[:, 1]) + 0.5 * X[:, 2] ** 2 - X[:, 6] * X[:, 7] + 5.0 * X[:, 0]).astype(np.float32)
+    net, stats = LP.train(X, Y, epochs=100, batch=1024, lr=3e-3, seed=0, log=print)
+    assert stats["val_rmse_log10"] < 0.1 * stats["val_base_rmse_log10"], stats
+    assert net.history[-1] < 0.05 * net.history[0]
+
+
+def test_pipeline_trains_and_beats_nested_on_time(tmp_path):
+    """Dataset from nested DWR -> device training -> save/load -> learned
+    indicators on a held-out problem: far cheaper than the nested estimate it
+    stands in for, and ranking elements like it."""
+    X, Y, G, runs = LP.make_dataset(4, seed=11, cube_cycles=2, bar_cycles=1, growth=3.0, log=print)
+    assert X.shape[0] == Y.shape[0] == G.shape[0] and len(runs) == 2 * 3 + 2 * 2
+    assert np.isfinite(X).all() and np.isfinite(Y).all()
+    net, stats = LP.train(X, Y, epochs=30, batch=1024, groups=G, log=print)
+    assert stats["val_rmse_log10"] < 0.5 * stats["val_base_rmse_log10"], stats
+    LP.save(net, tmp_path / "net.npz", meta=stats)
+    net2 = LP.load(tmp_path / "net.npz")
+    assert np.array_equal(net2.mlp.get()[0], net.mlp.get()[0]) and net2.meta["epochs"] == 30
+    p = LP.sample_problem(np.random.default_rng(777), "cube", 8)
+    m = LP.Model(p, *LP.base_mesh(p))
+    assert m.newton()["converged"]
+    d = LP.nested_dwr(m)
+    LP.learned_eta(m, net2, fallback=False)   # warm
+    l = LP.learned_eta(m, net2, fallback=False)
+    print("nested %.2f ms, learned %.3f ms, sums %.4g %.4g, rank corr %.3f" % (
+        d["seconds"] * 1e3, l["seconds"] * 1e3, d["eta"].sum(), l["eta"].sum(), LP._rank_corr(d["eta"], l["eta"])))
+    assert l["seconds"] * 10 < d["seconds"]
+    assert LP._rank_corr(d["eta"], l["eta"]) > 0.5
+
+
+def test_learned_falls_back_to_nested_out_of_range():
+    """An input box that excludes the mesh's features sends the estimate to
+    the plain nested DWR."""
+    p = _problem("cube", 4)
+    m = LP.Model(p, *LP.base_mesh(p))
+    m.newton()
+    norms = np.zeros(L.NNORMS, dtype=np.float32)
+    norms[12:24] = 1.0
+    norms[25] = 1.0
+    norms[L.OFF_LO:L.OFF_HI] = -np.inf
+    norms[L.OFF_HI:L.OFF_C] = np.inf
+    norms[L.OFF_LO + 10], norms[L.OFF_HI + 10] = 0.3, 0.4    # nu = 0.25 is outside
+    mlp = g.MLP()
+    mlp.set(None, norms)
+    out = LP.learned_eta(m, LP.Learned(mlp, norms))
+    assert out["fallback"] and out["outside"] == m.ne
+    ref = LP.nested_dwr(m)
+    # equal to rounding: the QoI derivative is scattered with FP64 atomics
+    assert np.allclose(out["eta"], ref["eta"], rtol=1e-10, atol=1e-14 * ref["eta"].max())
+
+
+@pytest.mark.skipif(not LP.DEFAULT_MODEL.exists(), reason="no packaged network")
+@pytest.mark.parametrize("geometry,n,seed", [("cube", 8, 31), ("bar", 4, 32)])
+def test_packaged_network_tracks_nested(geometry, n, seed):
+    """The packaged network (trained on 200 seeded runs) on unseen problems:
+    element ranking and indicator sum close to the nested DWR's, no fallback."""
+    net = LP.load(LP.DEFAULT_MODEL)
+    p = LP.sample_problem(np.random.default_rng(seed), geometry, n)
+    m = LP.Model(p, *LP.base_mesh(p))
+    assert m.newton()["converged"]
+    d = LP.nested_dwr(m)
+    l = LP.learned_eta(m, net)
+    rc = LP._rank_corr(d["eta"], l["eta"])
+    ratio = l["eta"].sum() / d["eta"].sum()
+    print(geometry, "rank corr %.3f, sum ratio %.3f, outside %d" % (rc, ratio, l["outside"]))
+    assert not l["fallback"]
+    assert rc > 0.7
+    assert 0.6 < ratio < 1.6
