@@ -323,6 +323,39 @@ def bench_batch(args) -> int:
     return 0
 
 
+def time_learned_kernels(m, net, reps=20):
+    """Device time of the two learned-indicator launches on the final mesh,
+    by HIP events on the library stream."""
+    import numpy as np
+
+    import goal_amd as g
+    from goal_amd import learned as LP
+
+    feat = LP.features(m)
+    y = g.DeviceBuffer(m.ne, np.float32)
+    net.mlp.forward(feat, m.ne, y)
+    g.sync()
+    g.event_record(10)
+    for _ in range(reps):
+        LP.features(m, feat)
+    g.event_record(11)
+    for _ in range(reps):
+        net.mlp.forward(feat, m.ne, y, count=False)
+    g.event_record(12)
+    g.sync()
+    f_ms, m_ms = g.event_elapsed(10, 11) / reps, g.event_elapsed(11, 12) / reps
+    n = m.ne
+    fwd_bytes = n * (4 * 12 + 4)             # feature rows in, predictions out
+    fwd_flops = n * 2 * (64 * 16 + 64 * 64)  # the two products as issued (K of layer 1 padded to 16)
+    log(f"learned kernels on {n} tets: features {f_ms:.3f} ms (with the AvgVM element pass), "
+        f"network {m_ms:.3f} ms = {fwd_bytes / (m_ms * 1e-3) / 1e9:.0f} GB/s, "
+        f"{fwd_flops / (m_ms * 1e-3) / 1e12:.1f} TFLOP/s bf16")
+    return {"elements": n, "features_ms": round(f_ms, 4), "forward_ms": round(m_ms, 4),
+            "forward_bytes": fwd_bytes, "forward_flops": fwd_flops,
+            "forward_GBps": round(fwd_bytes / (m_ms * 1e-3) / 1e9, 1),
+            "forward_TFLOPs": round(fwd_flops / (m_ms * 1e-3) / 1e12, 2)}
+
+
 def bench_learned(args) -> int:
     """Goal-oriented adaptation: learned indicator vs plain nested DWR."""
     import numpy as np
@@ -335,13 +368,20 @@ def bench_learned(args) -> int:
     train_info = None
     if args.retrain or not path.exists():
         t0 = time.perf_counter()
-        ckpt = Path("gpurun_out") / "learned_dataset.npz" if Path("gpurun_out").is_dir() else None
-        X, Y, G, runs = LP.make_dataset(args.train_runs, seed=0, checkpoint=ckpt, log=log)
+        if args.dataset:   # a dataset saved by an earlier run's checkpoint
+            with np.load(args.dataset, allow_pickle=False) as f:
+                X, Y, G = f["X"], f["Y"], f["G"]
+                runs = json.loads(bytes(f["runs"]).decode())
+        else:
+            ckpt = Path("gpurun_out") / "learned_dataset.npz" if Path("gpurun_out").is_dir() else None
+            X, Y, G, runs = LP.make_dataset(args.train_runs, seed=0, checkpoint=ckpt, log=log)
         data_s = time.perf_counter() - t0
         log(f"dataset: {X.shape[0]} samples from {args.train_runs} runs ({len(runs)} meshes) in {data_s:.1f} s")
         net, stats = LP.train(X, Y, epochs=40, batch=4096, lr=2e-3, seed=0, groups=G, log=log)
-        train_info = dict(stats, dataset_seconds=round(data_s, 1), runs=args.train_runs, meshes=len(runs),
-                          samples=int(X.shape[0]))
+        train_info = dict(stats, dataset_seconds=round(data_s, 1), runs=len({r["run"] for r in runs}),
+                          meshes=len(runs), samples=int(X.shape[0]),
+                          max_train_elems={geo: max(r["elems"] for r in runs if r["geometry"] == geo)
+                                           for geo in ("cube", "bar")})
         out_path = Path(args.save_model) if args.save_model else Path("gpurun_out") / "learned_avgvm.npz"
         out_path.parent.mkdir(parents=True, exist_ok=True)
         LP.save(net, out_path, meta=train_info)
@@ -367,6 +407,8 @@ def bench_learned(args) -> int:
             res[method] = recs
             if method == "nested":
                 ref = LP.overkill_qoi(last, log=log)
+            if method == "learned" and p.geometry == "cube":
+                kernels = time_learned_kernels(last, net)
             del last
         for method in methods:
             for r in res[method]:
@@ -405,9 +447,13 @@ def bench_learned(args) -> int:
                                f"1:8 refinement of the final nested-adapted mesh",
                    "final_elems": args.final_elems, "final_elems_bar": args.final_elems_bar,
                    "cycles": args.cycles},
-        "roofline": None,
+        "roofline": {"bound": "hbm", "achieved": kernels["forward_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kernels["forward_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "mlp_forward_kernel", "bf16_TFLOPs": kernels["forward_TFLOPs"],
+                     "bf16_mfma_frac": round(kernels["forward_TFLOPs"] / 2500.0, 5)},
         "cpu_baseline": None,
         "detail": {"estimate_seconds_nested": round(est_n, 4), "estimate_seconds_learned": round(est_l, 5),
+                   "learned_kernels": kernels,
                    "effectivity_pairs_same_mesh": [[round(a, 4), round(b, 4)] for a, b in same],
                    "mean_abs_effectivity_gap": round(float(np.mean(gaps)), 4) if gaps else None,
                    "max_abs_effectivity_gap": round(float(np.max(gaps)), 4) if gaps else None,
@@ -436,6 +482,7 @@ def main() -> int:
     ap.add_argument("--save-model", default="", help="learned: where a retrained network is written")
     ap.add_argument("--train-runs", type=int, default=200)
     ap.add_argument("--train-only", action="store_true", help="learned: stop after training")
+    ap.add_argument("--dataset", default="", help="learned: train from a saved dataset (.npz) instead")
     ap.add_argument("--cycles", type=int, default=5)
     ap.add_argument("--final-elems", type=int, default=2_000_000, help="learned: final size of the cube runs")
     ap.add_argument("--final-elems-bar", type=int, default=200_000, help="learned: final size of the bar runs")

@@ -402,13 +402,14 @@ class MLP:
         check(lib.goal_mlp_get(self.h, _ptr(pa), _ptr(wq), _ptr(na)))
         return pa, wq, na
 
-    def forward(self, x: DeviceBuffer, n: int, y: DeviceBuffer) -> int:
-        """y[:n] = predictions for rows x[:n]; returns rows outside the training box."""
+    def forward(self, x: DeviceBuffer, n: int, y: DeviceBuffer, count: bool = True) -> int | None:
+        """y[:n] = predictions for rows x[:n]; returns rows outside the
+        training box (count=False: asynchronous, returns None)."""
         if x.n < n * MLP_IN or y.n < n:
             raise ValueError("MLP.forward: buffers too small")
         out = C.c_int64(0)
-        check(lib.goal_mlp_forward(self.h, x.ptr, int(n), y.ptr, C.byref(out)))
-        return out.value
+        check(lib.goal_mlp_forward(self.h, x.ptr, int(n), y.ptr, C.byref(out) if count else None))
+        return out.value if count else None
 
     def gradient(self, x: DeviceBuffer, y: DeviceBuffer, idx: DeviceBuffer, n: int):
         loss = C.c_float(0.0)

@@ -204,41 +204,97 @@ __device__ void stage_net(SharedNet& S, const MlpArgs& a) {
   __syncthreads();
 }
 
-constexpr int kFwdThreads = 256;
+__device__ inline float round_bf16(float f) { return bf16_to_f32(f32_to_bf16(f)); }
 
-// y[i] = s_y * net((x[i] - mu) * is) + mu_y + c * x[i][0].  Rows outside the
-// training feature box [lo, hi] are counted in *ood.
+// ---------------------------------------------------------- forward (MFMA)
+//
+// Layers 1 and 2 run on the matrix cores as v_mfma_f32_16x16x16_bf16 with the
+// samples as the N dimension.  For lane l, r = l % 16 and q = l / 16:
+//   A operand  lane holds A[r][4q + i]      (weights: output row r of a block)
+//   B operand  lane holds B[4q + i][r]      (inputs of sample r)
+//   C/D        lane holds D[4q + i][r]      i = 0..3
+// so the D of a layer-1 row block is, after bias, ReLU and rounding, exactly
+// the B operand of layer 2 for the matching K block: the activations never
+// leave the registers.  All weights sit in registers for the whole launch.
+// The inputs of each product (normalized features, layer-1 activations) are
+// rounded to BF16; accumulation is FP32.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kFwdThreads = 256;
+constexpr int kFwdWaves = kFwdThreads / 64;
+
+__device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// y[i] = s_y * net(bf16((x[i] - mu) * is)) + mu_y + c * x[i][0].  Rows with a
+// feature outside the training box [lo, hi] are counted in *ood.
 __global__ __launch_bounds__(kFwdThreads) void mlp_forward_kernel(MlpArgs a, const float* x, int64_t n, float* y,
                                                                   int* ood) {
-  __shared__ SharedNet S;
-  stage_net(S, a);
-  const int64_t i = (int64_t)blockIdx.x * kFwdThreads + threadIdx.x;
-  if (i >= n) return;
-  float xs[kIn];
-  bool out = false;
-  const float x0 = x[i * kIn];
-  for (int k = 0; k < kIn; ++k) {
-    const float v = x[i * kIn + k];
-    out |= !(v >= a.norms[kMlpOffLo + k] && v <= a.norms[kMlpOffHi + k]);
-    xs[k] = (v - S.mu[k]) * S.is[k];
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  bf16x4 A1[4], A2[4][4];
+  float b1[4][4], b2[4][4], w3[4][4];
+  for (int mb = 0; mb < 4; ++mb) {
+    for (int i = 0; i < 4; ++i) {
+      const int k = 4 * q + i;
+      A1[mb][i] = k < kIn ? (short)a.wq[oW1 + (16 * mb + r) * kIn + k] : (short)0;
+      const int row = 16 * mb + 4 * q + i;   // D row held by this lane
+      b1[mb][i] = a.params[oB1 + row];
+      b2[mb][i] = a.params[oB2 + row];
+      w3[mb][i] = bf16_to_f32(a.wq[oW3 + row]);
+    }
+    for (int kb = 0; kb < 4; ++kb)
+      for (int i = 0; i < 4; ++i) A2[mb][kb][i] = (short)a.wq[oW2 + (16 * mb + r) * kHid + 16 * kb + 4 * q + i];
   }
-  float h1[kHid];
-#pragma unroll
-  for (int j = 0; j < kHid; ++j) {
-    float s = S.b1[j];
-#pragma unroll
-    for (int k = 0; k < kIn; ++k) s += S.w1[j * kIn + k] * xs[k];
-    h1[j] = fmaxf(s, 0.0f);
+  float mu[4], is[4], lo[4], hi[4];
+  for (int i = 0; i < 4; ++i) {
+    const int k = 4 * q + i;
+    mu[i] = k < kIn ? a.norms[k] : 0.0f;
+    is[i] = k < kIn ? a.norms[kIn + k] : 0.0f;
+    lo[i] = k < kIn ? a.norms[kMlpOffLo + k] : 0.0f;
+    hi[i] = k < kIn ? a.norms[kMlpOffHi + k] : 0.0f;
   }
-  float o = S.b3;
-  for (int jj = 0; jj < kHid; ++jj) {
-    float s = S.b2[jj];
-#pragma unroll
-    for (int j = 0; j < kHid; ++j) s += S.w2[jj * kHid + j] * h1[j];
-    o += S.w3[jj] * fmaxf(s, 0.0f);
+  const float b3 = a.params[oB3], mu_y = a.norms[2 * kIn], s_y = a.norms[2 * kIn + 1], cx = a.norms[kMlpOffC];
+  const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  const int64_t tiles = (n + 15) / 16;
+  const int64_t wave = (int64_t)blockIdx.x * kFwdWaves + (threadIdx.x >> 6);
+  const int64_t stride = (int64_t)gridDim.x * kFwdWaves;
+  for (int64_t t = wave; t < tiles; t += stride) {   // uniform per wave
+    const int64_t smp = t * 16 + r;
+    const bool live = smp < n;
+    bf16x4 B1;
+    int out = 0;
+    for (int i = 0; i < 4; ++i) {
+      const int k = 4 * q + i;
+      float v = 0.0f;
+      if (k < kIn && live) {
+        v = x[smp * kIn + k];
+        out |= !(v >= lo[i] && v <= hi[i]);
+      }
+      B1[i] = (short)f32_to_bf16(k < kIn ? (v - mu[i]) * is[i] : 0.0f);
+    }
+    bf16x4 B2[4];
+    for (int mb = 0; mb < 4; ++mb) {
+      const f32x4 z = mfma16(A1[mb], B1, zero);
+      for (int i = 0; i < 4; ++i) B2[mb][i] = (short)f32_to_bf16(fmaxf(z[i] + b1[mb][i], 0.0f));
+    }
+    float o = 0.0f;
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 z = zero;
+      for (int kb = 0; kb < 4; ++kb) z = mfma16(A2[mb][kb], B2[kb], z);
+      for (int i = 0; i < 4; ++i) o += w3[mb][i] * fmaxf(z[i] + b2[mb][i], 0.0f);
+    }
+    // sum the four row groups of sample r (lanes r, r+16, r+32, r+48)
+    o += __shfl_xor(o, 16, 64);
+    o += __shfl_xor(o, 32, 64);
+    out |= __shfl_xor(out, 16, 64);
+    out |= __shfl_xor(out, 32, 64);
+    if (q == 0 && live) {
+      y[smp] = (o + b3) * s_y + mu_y + cx * x[smp * kIn];
+      if (out && ood) atomicAdd(ood, 1);
+    }
   }
-  y[i] = o * S.s_y + S.mu_y + S.c * x0;
-  if (out && ood) atomicAdd(ood, 1);
 }
 
 // Gradient of L = (1/B) sum_i (net(x_i) - t_i)^2, t_i = (y_i - mu_y - c x_i0)/s_y, over
@@ -277,7 +333,7 @@ __global__ __launch_bounds__(kGradThreads) void mlp_grad_kernel(MlpArgs a, const
       float t = 0.0f;
       if (live) {
         const int64_t r = idx[row];
-        for (int k = 0; k < kIn; ++k) xs[k] = (x[r * kIn + k] - S.mu[k]) * S.is[k];
+        for (int k = 0; k < kIn; ++k) xs[k] = round_bf16((x[r * kIn + k] - S.mu[k]) * S.is[k]);
         t = (y[r] - S.mu_y - S.c * x[r * kIn]) / S.s_y;
       } else {
         for (int k = 0; k < kIn; ++k) xs[k] = 0.0f;
@@ -289,7 +345,7 @@ __global__ __launch_bounds__(kGradThreads) void mlp_grad_kernel(MlpArgs a, const
         float s = S.b1[j];
 #pragma unroll
         for (int k = 0; k < kIn; ++k) s += S.w1[j * kIn + k] * xs[k];
-        T.h1[tid][j] = fmaxf(s, 0.0f);
+        T.h1[tid][j] = round_bf16(fmaxf(s, 0.0f));
       }
       float o = S.b3;
       for (int jj = 0; jj < kHid; ++jj) {
@@ -407,7 +463,9 @@ hipError_t launch_features(const FeatureArgs& a, hipStream_t s) {
 
 hipError_t launch_mlp_forward(const MlpArgs& a, const float* x, int64_t n, float* y, int* ood, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  const unsigned g = (unsigned)((n + kFwdThreads - 1) / kFwdThreads);
+  // persistent: each wave loads the weights once, then walks 16-row tiles
+  const int64_t tiles = (n + 15) / 16;
+  const unsigned g = (unsigned)std::min<int64_t>((tiles + kFwdWaves - 1) / kFwdWaves, 2048);
   hipLaunchKernelGGL(mlp_forward_kernel, dim3(g), dim3(kFwdThreads), 0, s, a, x, n, y, ood);
   return hipGetLastError();
 }

@@ -167,6 +167,8 @@ def nested_dwr(m: Model, t=1.0, lin_rtol=1e-8, lin_max=40000, restart=100) -> di
     nested.transfer_states(m.states, st)
     si = g.SolInfo(fine)
     n = fine.num_dofs
+    g.sync()
+    t1 = time.perf_counter()
     si.zero_all()
     mech.jacobian(si, adjoint=True, save_states=False)
     mech.qoi(AVG_VM, si, elem_set=QOI_SET)
@@ -175,7 +177,10 @@ def nested_dwr(m: Model, t=1.0, lin_rtol=1e-8, lin_max=40000, restart=100) -> di
     R, dMdu, _ = si.dev()
     z = g.DeviceArray(n)
     z.fill(0.0)
+    g.sync()
+    t2 = time.perf_counter()
     k, rr = si.solve(dMdu, z, rtol=lin_rtol, max_iters=lin_max, restart=restart)
+    t3 = time.perf_counter()
     zR = si.dot_R(z.ptr.value)
     zc = g.DeviceArray(n)
     g.device_copy(zc, z, n)
@@ -194,8 +199,11 @@ def nested_dwr(m: Model, t=1.0, lin_rtol=1e-8, lin_max=40000, restart=100) -> di
     nested.set_error(eta_f, eta_b)
     eta = eta_b.numpy()
     g.sync()
+    t4 = time.perf_counter()
     return {"eta": eta, "bound": bound, "zR": zR, "adjoint_iters": k, "adjoint_relres": rr,
-            "fine_elems": fine.num_elems, "seconds": time.perf_counter() - t0}
+            "fine_elems": fine.num_elems, "seconds": t4 - t0,
+            "split": {"refine_and_setup": t1 - t0, "adjoint_assembly": t2 - t1, "adjoint_solve": t3 - t2,
+                      "localize": t4 - t3}}
 
 
 @dataclass
@@ -230,6 +238,8 @@ def learned_eta(m: Model, net: Learned, fallback: bool = True) -> dict:
 
 def adapt(m: Model, eta, target: int, split_ratio: float = 1.5, max_passes: int = 8) -> Model:
     """Size field + refinement; the new model starts from the transferred solution."""
+    if not np.all(np.isfinite(eta)):
+        raise ValueError("adapt: non-finite indicators")
     size = g.size_field(m.coords, m.conn, eta, int(target))
     A = g.Adaptation(m.coords, m.conn, size, split_ratio, max_passes)
     new = Model(m.p, A.coords, A.conn)
@@ -255,35 +265,50 @@ def make_dataset(num_runs: int, seed: int = 0, cube_cycles: int = 4, bar_cycles:
         return (np.concatenate(Xs).astype(np.float32), np.concatenate(Ys).astype(np.float32),
                 np.concatenate(Gs), runs)
 
+    skipped = []
     for r in range(num_runs):
         rng = np.random.default_rng([seed, r])
         geometry = "cube" if r % 2 == 0 else "bar"
         n = int(rng.choice([4, 8])) if geometry == "cube" else 4
         cycles = cube_cycles if geometry == "cube" else bar_cycles
         p = sample_problem(rng, geometry, n)
-        m = Model(p, *base_mesh(p))
-        info = m.newton(lin_rtol=lin_rtol)
-        for c in range(cycles + 1):
-            d = nested_dwr(m, lin_rtol=lin_rtol)
-            X = features(m).numpy(m.ne * g.MLP_IN).reshape(m.ne, g.MLP_IN)
-            keep = np.nonzero(d["eta"] > 0)[0]
-            if keep.size > cap:
-                keep = np.sort(rng.choice(keep, cap, replace=False))
-            Xs.append(X[keep])
-            Ys.append(np.log10(d["eta"][keep] / p.E))
-            Gs.append(np.full(keep.size, len(runs), dtype=np.int32))
-            runs.append({"run": r, "cycle": c, "geometry": geometry, "elems": m.ne, "newton": info["iters"],
-                         "converged": info["converged"], "eta_sum": float(d["eta"].sum()), "zR": d["zR"],
-                         "nested_seconds": round(d["seconds"], 4)})
-            if c < cycles:
-                m = adapt(m, d["eta"], int(m.ne * growth))
-                info = m.newton(lin_rtol=lin_rtol)
+        try:
+            _run_into(p, r, cycles, growth, cap, lin_rtol, rng, Xs, Ys, Gs, runs)
+        except (g.GoalError, ValueError) as e:   # a failed solve ends the run; its earlier meshes stay
+            skipped.append(r)
+            if log:
+                log(f"[learned] run {r} stopped: {e}")
         if (r + 1) % 10 == 0 and log:
             log(f"[learned] dataset: {r + 1}/{num_runs} runs, {sum(len(y) for y in Ys)} samples")
         if checkpoint is not None and (r + 1) % 25 == 0:
             X_, Y_, G_, _ = arrays()
             np.savez(checkpoint, X=X_, Y=Y_, G=G_, runs=np.frombuffer(json.dumps(runs).encode(), dtype=np.uint8))
     return arrays()
+
+
+def _run_into(p, r, cycles, growth, cap, lin_rtol, rng, Xs, Ys, Gs, runs):
+    """One training run: solve, nested DWR, adapt, `cycles` times."""
+    m = Model(p, *base_mesh(p))
+    info = m.newton(lin_rtol=lin_rtol)
+    for c in range(cycles + 1):
+        if not info["converged"]:
+            raise ValueError(f"Newton did not converge on {m.ne} tets (|R| = {info['norm']:.3e})")
+        d = nested_dwr(m, lin_rtol=lin_rtol)
+        if not np.all(np.isfinite(d["eta"])):
+            raise ValueError(f"non-finite nested indicators on {m.ne} tets")
+        X = features(m).numpy(m.ne * g.MLP_IN).reshape(m.ne, g.MLP_IN)
+        keep = np.nonzero(d["eta"] > 0)[0]
+        if keep.size > cap:
+            keep = np.sort(rng.choice(keep, cap, replace=False))
+        Xs.append(X[keep])
+        Ys.append(np.log10(d["eta"][keep] / p.E))
+        Gs.append(np.full(keep.size, len(runs), dtype=np.int32))
+        runs.append({"run": r, "cycle": c, "geometry": p.geometry, "elems": m.ne, "newton": info["iters"],
+                     "converged": info["converged"], "eta_sum": float(d["eta"].sum()), "zR": d["zR"],
+                     "nested_seconds": round(d["seconds"], 4)})
+        if c < cycles:
+            m = adapt(m, d["eta"], int(m.ne * growth))
+            info = m.newton(lin_rtol=lin_rtol)
 
 
 def fit_norms(X: np.ndarray, Y: np.ndarray, margin: float = 0.05) -> np.ndarray:
@@ -419,7 +444,8 @@ def run_cycles(p: Problem, method: str, cycles: int, final_elems: int, net: Lear
         if c < cycles:
             if method == "nested":
                 d = nested_dwr(m)
-                rec.update(zR=d["zR"], bound=d["bound"], adjoint_iters=d["adjoint_iters"])
+                rec.update(zR=d["zR"], bound=d["bound"], adjoint_iters=d["adjoint_iters"],
+                           nested_split={k: round(v, 4) for k, v in d["split"].items()})
                 if net is not None:   # the network on the same mesh, for a like-for-like comparison
                     l = learned_eta(m, net, fallback=False)
                     rec.update(learned_eta_sum=float(np.sum(l["eta"])), learned_seconds=l["seconds"],

@@ -8,9 +8,9 @@ adaptation size field:
   state (no reference counterpart; von Mises as src/goal_von_mises.cpp:7-17
   enters through the per-element integral the QoI kernel returns).
 * bf16, mlp_forward, mlp_loss_grad, adam: the 12 -> 64 -> 64 -> 1 ReLU network
-  with BF16 weights and FP32 biases (no reference counterpart).  The gradient
-  is back-propagation written out by hand; tests pin it with central finite
-  differences.
+  with BF16 weights, BF16-rounded inputs to its two matrix products and FP32
+  biases (no reference counterpart).  The gradient is back-propagation written
+  out by hand; tests pin it with central finite differences and torch autograd.
 """
 from __future__ import annotations
 
@@ -121,32 +121,42 @@ def unpack(params, rounded=True):
     return W1, b1, W2, b2, W3, b3
 
 
-def _layers(net, norms, X):
+def _layers(net, norms, X, rounded=True):
+    """Forward pass.  With `rounded`, the inputs of the two matrix products
+    are BF16-rounded as on the device: the normalized features (computed in
+    float32) and the layer-1 activations."""
     W1, b1, W2, b2, W3, b3 = net
-    nrm = np.asarray(norms, dtype=np.float64)
-    xs = (np.asarray(X, dtype=np.float64) - nrm[:IN]) * nrm[IN:2 * IN]
+    if rounded:
+        n32 = np.asarray(norms, dtype=np.float32)
+        xs = bf16((np.asarray(X, dtype=np.float32) - n32[:IN]) * n32[IN:2 * IN]).astype(np.float64)
+    else:
+        nrm = np.asarray(norms, dtype=np.float64)
+        xs = (np.asarray(X, dtype=np.float64) - nrm[:IN]) * nrm[IN:2 * IN]
     z1 = xs @ W1.T + b1
     h1 = np.maximum(z1, 0.0)
+    if rounded:
+        h1 = bf16(h1.astype(np.float32)).astype(np.float64)
     z2 = h1 @ W2.T + b2
     h2 = np.maximum(z2, 0.0)
     o = h2 @ W3 + b3
-    return xs, h1, h2, o
+    return xs, z1, h1, h2, o
 
 
 def mlp_forward(params, norms, X, rounded=True):
     """y = net((x - mu) * is) * s_y + mu_y + c * x[0]."""
     nrm = np.asarray(norms, dtype=np.float64)
-    _, _, _, o = _layers(unpack(params, rounded), norms, X)
+    o = _layers(unpack(params, rounded), norms, X, rounded)[-1]
     return o * nrm[2 * IN + 1] + nrm[2 * IN] + nrm[OFF_C] * np.asarray(X, dtype=np.float64)[:, 0]
 
 
 def mlp_loss_grad(params, norms, X, Y, rounded=True):
     """L = mean_i (net(x_i) - t_i)^2 with t_i = (y_i - mu_y - c x_i0)/s_y, and
-    dL/dparams (with respect to the weights as used)."""
+    dL/dparams (with respect to the weights as used; the BF16 rounding of the
+    activations passes gradients straight through)."""
     net = unpack(params, rounded)
     W1, b1, W2, b2, W3, b3 = net
     nrm = np.asarray(norms, dtype=np.float64)
-    xs, h1, h2, o = _layers(net, norms, X)
+    xs, z1, h1, h2, o = _layers(net, norms, X, rounded)
     t = (np.asarray(Y, dtype=np.float64) - nrm[2 * IN] - nrm[OFF_C] * np.asarray(X, dtype=np.float64)[:, 0]) / nrm[
         2 * IN + 1]
     B = xs.shape[0]
@@ -154,7 +164,7 @@ def mlp_loss_grad(params, norms, X, Y, rounded=True):
     loss = float(np.mean(r * r))
     d3 = 2.0 * r / B                       # dL/do
     d2 = np.outer(d3, W3) * (h2 > 0)       # dL/dz2
-    d1 = (d2 @ W2) * (h1 > 0)              # dL/dz1
+    d1 = (d2 @ W2) * (z1 > 0)              # dL/dz1
     g = np.zeros(NPARAMS)
     g[OFF_W1:OFF_B1] = (d1.T @ xs).ravel()
     g[OFF_B1:OFF_W2] = d1.sum(0)

@@ -3,9 +3,11 @@ network's forward pass, its gradient and the Adam step against the float64
 oracle (oracle/learned.py), bitwise reproducibility of training, and the
 adaptation pipeline that uses them.
 
-Tolerances: features are FP32 roundings of FP64 values (2e-6); the network
-accumulates 4,928 FP32 products per row (forward 1e-5 of the output scale,
-gradient 1e-4 normwise).
+Tolerances: features are FP32 roundings of FP64 values (2e-6).  The network
+rounds the inputs of its two products to BF16 and accumulates in FP32, in
+another order than the float64 oracle, so an activation on a BF16 rounding
+boundary may round the other way: forward 1e-5 of the output scale for 99% of
+rows (5e-3 at most), gradient 1e-3 normwise.
 """
 from __future__ import annotations
 
@@ -74,7 +76,10 @@ def test_mlp_forward_matches_oracle():
     outside = mlp.forward(xd, n, yd)
     ref = L.mlp_forward(params, norms, X)
     scale = np.abs(ref).max()
-    assert np.max(np.abs(yd.numpy() - ref)) <= 1e-5 * scale
+    d = np.abs(yd.numpy() - ref)
+    # FP32 matrix-core accumulation vs float64: a layer-1 activation that sits
+    # on a BF16 rounding boundary can round the other way (one BF16 ulp)
+    assert np.mean(d <= 1e-5 * scale) > 0.99 and d.max() <= 5e-3 * scale, (np.sort(d)[-5:], scale)
     assert outside == int(np.count_nonzero((X[:, 3] < -1.0) | (X[:, 3] > 1.5)))
 
 
@@ -88,10 +93,10 @@ def test_mlp_gradient_matches_oracle():
     yd = g.DeviceBuffer.from_host(Y, np.float32)
     loss, grad = mlp.gradient(xd, yd, g.DeviceBuffer.from_host(idx, np.int32), idx.size)
     rl, rg = L.mlp_loss_grad(params, norms, X[idx], Y[idx])
-    assert abs(loss - rl) <= 1e-5 * rl
-    assert np.linalg.norm(grad - rg) <= 1e-4 * np.linalg.norm(rg)
+    assert abs(loss - rl) <= 1e-4 * rl
+    assert np.linalg.norm(grad - rg) <= 1e-3 * np.linalg.norm(rg)
     for lo, hi in ((L.OFF_W1, L.OFF_B1), (L.OFF_W2, L.OFF_B2), (L.OFF_W3, L.OFF_B3 + 1)):
-        assert np.linalg.norm(grad[lo:hi] - rg[lo:hi]) <= 1e-4 * np.linalg.norm(rg[lo:hi])
+        assert np.linalg.norm(grad[lo:hi] - rg[lo:hi]) <= 1e-3 * np.linalg.norm(rg[lo:hi])
 
 
 def test_adam_step_matches_oracle_and_training_is_reproducible():

@@ -93,8 +93,14 @@ def test_mlp_forward_and_gradient_match_torch_autograd():
                               for a in L.unpack(p, rounded=True))
     nrm = torch.tensor(norms.astype(np.float64))
     x = torch.tensor(X)
-    xs = (x - nrm[:12]) * nrm[12:24]
-    o = torch.relu(torch.relu(xs @ W1.T + b1) @ W2.T + b2) @ W3 + b3
+    # the device rounds the matrix-product inputs to BF16 (torch's own
+    # conversion here); gradients pass straight through the rounding
+    xs32 = torch.tensor(X, dtype=torch.float32)
+    n32 = torch.tensor(norms[:24], dtype=torch.float32)
+    xs = ((xs32 - n32[:12]) * n32[12:24]).to(torch.bfloat16).to(torch.float64)
+    h1 = torch.relu(xs @ W1.T + b1)
+    h1 = h1 + (h1.to(torch.float32).to(torch.bfloat16).to(torch.float64) - h1).detach()
+    o = torch.relu(h1 @ W2.T + b2) @ W3 + b3
     y = o * nrm[25] + nrm[24] + nrm[L.OFF_C] * x[:, 0]
     assert np.allclose(L.mlp_forward(p, norms, X), y.detach().numpy(), rtol=1e-13, atol=1e-13)
     t = (torch.tensor(Y) - nrm[24] - nrm[L.OFF_C] * x[:, 0]) / nrm[25]
