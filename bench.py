@@ -397,7 +397,9 @@ def bench_learned(args) -> int:
     rng = np.random.default_rng(args.eval_seed)
     problems = [(LP.sample_problem(rng, "cube", 8), args.final_elems),
                 (LP.sample_problem(rng, "bar", 4), args.final_elems_bar)]
+    problems = [pf for pf in problems if pf[0].geometry in args.problems.split(",")]
     report = []
+    kernels = None
     methods = ("nested", "learned", "uniform")
     for p, final in problems:
         log(f"problem: {p}, final {final} tets")
@@ -447,10 +449,11 @@ def bench_learned(args) -> int:
                                f"1:8 refinement of the final nested-adapted mesh",
                    "final_elems": args.final_elems, "final_elems_bar": args.final_elems_bar,
                    "cycles": args.cycles},
-        "roofline": {"bound": "hbm", "achieved": kernels["forward_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kernels["forward_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "mlp_forward_kernel", "bf16_TFLOPs": kernels["forward_TFLOPs"],
-                     "bf16_mfma_frac": round(kernels["forward_TFLOPs"] / 2500.0, 5)},
+        "roofline": None if kernels is None else {
+            "bound": "hbm", "achieved": kernels["forward_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(kernels["forward_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "mlp_forward_kernel", "bf16_TFLOPs": kernels["forward_TFLOPs"],
+            "bf16_mfma_frac": round(kernels["forward_TFLOPs"] / 2500.0, 5)},
         "cpu_baseline": None,
         "detail": {"estimate_seconds_nested": round(est_n, 4), "estimate_seconds_learned": round(est_l, 5),
                    "learned_kernels": kernels,
@@ -487,6 +490,7 @@ def main() -> int:
     ap.add_argument("--final-elems", type=int, default=2_000_000, help="learned: final size of the cube runs")
     ap.add_argument("--final-elems-bar", type=int, default=200_000, help="learned: final size of the bar runs")
     ap.add_argument("--eval-seed", type=int, default=2024)
+    ap.add_argument("--problems", default="cube,bar", help="learned: which evaluation problems to run")
     ap.add_argument("--blocks", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="hexahedra per side of the CPU baseline sample")

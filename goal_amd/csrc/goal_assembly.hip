@@ -615,8 +615,10 @@ int occupancy() {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &n, reinterpret_cast<const void*>(&element_kernel<NN, NQ, MODEL, KIND, TE>), Layout<NN, NQ, MODEL>::T,
             0) != hipSuccess ||
-        n < 1)
+        n < 1) {
       n = 1;
+      (void)hipGetLastError();  // the fallback stands; keep the error off the next launch
+    }
     return n;
   }();
   return occ;
@@ -823,8 +825,10 @@ int gather_grid(int order, bool f32, int cap, int64_t num_ranges) {
                                     : reinterpret_cast<const void*>(&gather_kernel<10, double>));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, kGatherThreads, sizeof(double) * (size_t)cap) !=
           hipSuccess ||
-      occ < 1)
+      occ < 1) {
     occ = 1;
+    (void)hipGetLastError();  // the fallback stands; keep the error off the next launch
+  }
   return (int)std::max<int64_t>(1, std::min<int64_t>(num_ranges, (int64_t)cus * occ));
 }
 

@@ -48,7 +48,22 @@ void hip_check(hipError_t e, const char* what) {
     throw HipError(GOAL_ERR_ALLOC, std::string(what) + ": " + hipGetErrorString(e));
   throw HipError(GOAL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
-#define HIPCHK(x) hip_check((x), #x)
+// A HIP call whose status is deliberately ignored (a free in a destructor, an
+// occupancy query with a fallback) still leaves its error in the thread's
+// last-error slot, where the hipGetLastError() after the next kernel launch
+// would report it as that launch's failure.  Each checked call therefore
+// first takes any such leftover out of the slot, naming it on stderr.
+void clear_stale_error(const char* before) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    std::fprintf(stderr, "goal_amd: an unchecked HIP call failed earlier (%s); seen before %s\n",
+                 hipGetErrorString(e), before);
+}
+#define HIPCHK(x)              \
+  do {                         \
+    clear_stale_error(#x);     \
+    hip_check((x), #x);        \
+  } while (0)
 
 // Every entry point runs its body through guard(): exceptions become status
 // codes plus a message, never an abort.
