@@ -356,6 +356,62 @@ def time_learned_kernels(m, net, reps=20):
             "forward_TFLOPs": round(fwd_flops / (m_ms * 1e-3) / 1e12, 2)}
 
 
+def bench_mlp(args) -> int:
+    """The learned indicator's network alone: forward passes over 2 M rows of
+    synthetic features drawn around the packaged network's normalization."""
+    import numpy as np
+
+    import goal_amd as g
+    from goal_amd import learned as LP
+
+    net = LP.load(Path(args.model) if args.model else LP.DEFAULT_MODEL)
+    n = args.rows
+    rng = np.random.default_rng(7)
+    mu, inv = net.norms[:12], net.norms[12:24]
+    X = (mu + rng.standard_normal((n, 12)) / inv).astype(np.float32)
+    xd = g.DeviceBuffer.from_host(X.ravel(), np.float32)
+    yd = g.DeviceBuffer(n, np.float32)
+    for _ in range(args.warmup):
+        net.mlp.forward(xd, n, yd, count=False)
+    g.sync()
+    K = args.steps
+    g.event_record(0)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        net.mlp.forward(xd, n, yd, count=False)
+    g.event_record(1)
+    g.sync()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / K
+    ev_ms = g.event_elapsed(0, 1) / K
+    nbytes = n * (4 * 12 + 4)
+    flops = n * 2 * (64 * 16 + 64 * 64)
+    gbs = nbytes / (ev_ms * 1e-3) / 1e9
+    tfs = flops / (ev_ms * 1e-3) / 1e12
+    log(f"{n} rows: {ev_ms * 1e3:.1f} us per forward pass (events), {gbs:.0f} GB/s, {tfs:.1f} TFLOP/s bf16")
+    out = {
+        "metric": "learned-indicator network forward rows/s",
+        "value": round(n / (wall_ms * 1e-3), 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 weights and product inputs, f32 accumulation",
+        "data": "synthetic (Gaussian features around the packaged normalization)",
+        "config": {"workload": f"12-64-64-1 network forward, {n} rows, v_mfma_f32_16x16x16_bf16"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "mlp_forward_kernel",
+                     "bf16_TFLOPs": round(tfs, 2), "bf16_mfma_frac": round(tfs / 2500.0, 5)},
+        "cpu_baseline": None,
+        "detail": {"event_ms": round(ev_ms, 5), "bytes_per_pass": nbytes, "flops_per_pass": flops},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_learned(args) -> int:
     """Goal-oriented adaptation: learned indicator vs plain nested DWR."""
     import numpy as np
@@ -476,10 +532,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="hexahedra per cube side")
     ap.add_argument("--order", type=int, default=2)
-    ap.add_argument("--workload", default="cube", choices=("cube", "bar", "batch", "learned"),
+    ap.add_argument("--workload", default="cube", choices=("cube", "bar", "batch", "learned", "mlp"),
                     help="cube: P2 neo-Hookean cube (the headline); bar: P1 J2 bar, 5.3 M tets; "
                          "batch: 64 meshes of uneven size in one graph launch; learned: adaptation with the "
-                         "learned indicator against plain nested DWR")
+                         "learned indicator against plain nested DWR; mlp: its network's forward pass alone")
+    ap.add_argument("--rows", type=int, default=2_000_000, help="mlp: rows per forward pass")
     ap.add_argument("--model", default="", help="learned: trained network (.npz); default the packaged one")
     ap.add_argument("--retrain", action="store_true", help="learned: build the dataset and train first")
     ap.add_argument("--save-model", default="", help="learned: where a retrained network is written")
@@ -531,6 +588,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "mlp":
+        return bench_mlp(args)
     wl = workload(args.workload, args.n, args.order)
     order, quad = wl["order"], wl["quad"]
     n = args.n

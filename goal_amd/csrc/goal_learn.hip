@@ -223,6 +223,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kFwdThreads = 256;
 constexpr int kFwdWaves = kFwdThreads / 64;
+constexpr int kFwdUnroll = 4;
 
 __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -233,19 +234,24 @@ __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 __global__ __launch_bounds__(kFwdThreads) void mlp_forward_kernel(MlpArgs a, const float* x, int64_t n, float* y,
                                                                   int* ood) {
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  // the BF16 weights, staged through LDS with coalesced loads; each wave
+  // then reads its operand fragments from there
+  __shared__ uint16_t sw[kP + 1];
+  for (int i = threadIdx.x; i < kP; i += kFwdThreads) sw[i] = a.wq[i];
+  __syncthreads();
   bf16x4 A1[4], A2[4][4];
   float b1[4][4], b2[4][4], w3[4][4];
   for (int mb = 0; mb < 4; ++mb) {
     for (int i = 0; i < 4; ++i) {
       const int k = 4 * q + i;
-      A1[mb][i] = k < kIn ? (short)a.wq[oW1 + (16 * mb + r) * kIn + k] : (short)0;
+      A1[mb][i] = k < kIn ? (short)sw[oW1 + (16 * mb + r) * kIn + k] : (short)0;
       const int row = 16 * mb + 4 * q + i;   // D row held by this lane
       b1[mb][i] = a.params[oB1 + row];
       b2[mb][i] = a.params[oB2 + row];
-      w3[mb][i] = bf16_to_f32(a.wq[oW3 + row]);
+      w3[mb][i] = bf16_to_f32(sw[oW3 + row]);
     }
     for (int kb = 0; kb < 4; ++kb)
-      for (int i = 0; i < 4; ++i) A2[mb][kb][i] = (short)a.wq[oW2 + (16 * mb + r) * kHid + 16 * kb + 4 * q + i];
+      for (int i = 0; i < 4; ++i) A2[mb][kb][i] = (short)sw[oW2 + (16 * mb + r) * kHid + 16 * kb + 4 * q + i];
   }
   float mu[4], is[4], lo[4], hi[4];
   for (int i = 0; i < 4; ++i) {
@@ -260,41 +266,54 @@ __global__ __launch_bounds__(kFwdThreads) void mlp_forward_kernel(MlpArgs a, con
   const int64_t tiles = (n + 15) / 16;
   const int64_t wave = (int64_t)blockIdx.x * kFwdWaves + (threadIdx.x >> 6);
   const int64_t stride = (int64_t)gridDim.x * kFwdWaves;
-  for (int64_t t = wave; t < tiles; t += stride) {   // uniform per wave
-    const int64_t smp = t * 16 + r;
-    const bool live = smp < n;
-    bf16x4 B1;
-    int out = 0;
-    for (int i = 0; i < 4; ++i) {
-      const int k = 4 * q + i;
-      float v = 0.0f;
-      if (k < kIn && live) {
-        v = x[smp * kIn + k];
-        out |= !(v >= lo[i] && v <= hi[i]);
+  int outside = 0;  // rows of this lane outside the box (lanes q == 0 only)
+  // kFwdUnroll consecutive tiles per iteration: their feature loads are all
+  // issued before the first product, so each wave keeps several in flight.
+  for (int64_t t0 = wave * kFwdUnroll; t0 < tiles; t0 += stride * kFwdUnroll) {   // uniform per wave
+    float v[kFwdUnroll][4];
+#pragma unroll
+    for (int u = 0; u < kFwdUnroll; ++u) {
+      const int64_t smp = (t0 + u) * 16 + r;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[u][i] = (4 * q + i < kIn && smp < n) ? x[smp * kIn + 4 * q + i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kFwdUnroll; ++u) {
+      const int64_t smp = (t0 + u) * 16 + r;
+      const bool live = smp < n;
+      bf16x4 B1;
+      int out = 0;
+      for (int i = 0; i < 4; ++i) {
+        const int k = 4 * q + i;
+        if (k < kIn && live) out |= !(v[u][i] >= lo[i] && v[u][i] <= hi[i]);
+        B1[i] = (short)f32_to_bf16(k < kIn ? (v[u][i] - mu[i]) * is[i] : 0.0f);
       }
-      B1[i] = (short)f32_to_bf16(k < kIn ? (v - mu[i]) * is[i] : 0.0f);
-    }
-    bf16x4 B2[4];
-    for (int mb = 0; mb < 4; ++mb) {
-      const f32x4 z = mfma16(A1[mb], B1, zero);
-      for (int i = 0; i < 4; ++i) B2[mb][i] = (short)f32_to_bf16(fmaxf(z[i] + b1[mb][i], 0.0f));
-    }
-    float o = 0.0f;
-    for (int mb = 0; mb < 4; ++mb) {
-      f32x4 z = zero;
-      for (int kb = 0; kb < 4; ++kb) z = mfma16(A2[mb][kb], B2[kb], z);
-      for (int i = 0; i < 4; ++i) o += w3[mb][i] * fmaxf(z[i] + b2[mb][i], 0.0f);
-    }
-    // sum the four row groups of sample r (lanes r, r+16, r+32, r+48)
-    o += __shfl_xor(o, 16, 64);
-    o += __shfl_xor(o, 32, 64);
-    out |= __shfl_xor(out, 16, 64);
-    out |= __shfl_xor(out, 32, 64);
-    if (q == 0 && live) {
-      y[smp] = (o + b3) * s_y + mu_y + cx * x[smp * kIn];
-      if (out && ood) atomicAdd(ood, 1);
+      bf16x4 B2[4];
+      for (int mb = 0; mb < 4; ++mb) {
+        const f32x4 z = mfma16(A1[mb], B1, zero);
+        for (int i = 0; i < 4; ++i) B2[mb][i] = (short)f32_to_bf16(fmaxf(z[i] + b1[mb][i], 0.0f));
+      }
+      float o = 0.0f;
+      for (int mb = 0; mb < 4; ++mb) {
+        f32x4 z = zero;
+        for (int kb = 0; kb < 4; ++kb) z = mfma16(A2[mb][kb], B2[kb], z);
+        for (int i = 0; i < 4; ++i) o += w3[mb][i] * fmaxf(z[i] + b2[mb][i], 0.0f);
+      }
+      // sum the four row groups of sample r (lanes r, r+16, r+32, r+48)
+      o += __shfl_xor(o, 16, 64);
+      o += __shfl_xor(o, 32, 64);
+      out |= __shfl_xor(out, 16, 64);
+      out |= __shfl_xor(out, 32, 64);
+      // feature 0 of sample r is v[u][0] of lane r (q == 0)
+      if (q == 0 && live) {
+        y[smp] = (o + b3) * s_y + mu_y + cx * v[u][0];
+        outside += out ? 1 : 0;
+      }
     }
   }
+  // one atomic per wave: a row-by-row count on a single address serializes
+  for (int off = 32; off > 0; off >>= 1) outside += __shfl_xor(outside, off, 64);
+  if (lane == 0 && outside && ood) atomicAdd(ood, outside);
 }
 
 // Gradient of L = (1/B) sum_i (net(x_i) - t_i)^2, t_i = (y_i - mu_y - c x_i0)/s_y, over
@@ -463,9 +482,23 @@ hipError_t launch_features(const FeatureArgs& a, hipStream_t s) {
 
 hipError_t launch_mlp_forward(const MlpArgs& a, const float* x, int64_t n, float* y, int* ood, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  // persistent: each wave loads the weights once, then walks 16-row tiles
+  // persistent: exactly the workgroups that are resident at once; each wave
+  // loads the weights once, then walks groups of 16-row tiles
+  static const int resident = [] {
+    int occ = 0, dev = 0, cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&mlp_forward_kernel),
+                                                     kFwdThreads, 0) != hipSuccess ||
+        occ < 1) {
+      occ = 1;
+      (void)hipGetLastError();
+    }
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return occ * cus;
+  }();
   const int64_t tiles = (n + 15) / 16;
-  const unsigned g = (unsigned)std::min<int64_t>((tiles + kFwdWaves - 1) / kFwdWaves, 2048);
+  const int64_t groups = (tiles + kFwdUnroll - 1) / kFwdUnroll;
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((groups + kFwdWaves - 1) / kFwdWaves, resident));
   hipLaunchKernelGGL(mlp_forward_kernel, dim3(g), dim3(kFwdThreads), 0, s, a, x, n, y, ood);
   return hipGetLastError();
 }
