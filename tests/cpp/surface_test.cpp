@@ -5,6 +5,12 @@
 // meshes this program is run on.  Prints one JSON object.
 //
 // usage: surface_test <order> <quad_order> <model> <n> <steps> [tangent bits: 64 | 32] [solver: lu | gmres]
+//                     [extra: adapt | learned:<file>]
+//   adapt            after the load steps (P1), adapt once with the nested indicators
+//                    (get_iso_target_size, adapt) and solve the last step again there
+//   learned:<file>   the learned indicator on the final state; <file> holds the
+//                    network as raw float32: GOAL_MLP_PARAMS parameters, then
+//                    GOAL_MLP_NORMS normalization values
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -64,6 +70,7 @@ int main(int argc, char** argv) {
   const int n = std::atoi(argv[4]), steps = std::atoi(argv[5]);
   const int tangent = argc > 6 ? std::atoi(argv[6]) : 64;
   const bool gmres = argc > 7 && std::string(argv[7]) == "gmres";
+  const std::string extra = argc > 8 ? argv[8] : "";
   try {
     auto disc = goal::Disc::box(n, n, n, 1.0, 1.0, 1.0, order, 0.1, 11);
     goal::MechanicsParams mp;
@@ -105,6 +112,46 @@ int main(int argc, char** argv) {
       double s = 0.0;
       for (double e : eta) s += e;
       std::printf(", \"eta_sum\": %.17g, \"bound\": %.17g, \"zR\": %.17g", s, adj.bound(), adj.z_dot_R());
+      if (extra.rfind("learned:", 0) == 0) {
+        std::vector<float> buf(GOAL_MLP_PARAMS + GOAL_MLP_NORMS);
+        FILE* f = std::fopen(extra.substr(8).c_str(), "rb");
+        if (!f || std::fread(buf.data(), sizeof(float), buf.size(), f) != buf.size()) {
+          std::fprintf(stderr, "cannot read %s\n", extra.substr(8).c_str());
+          return 2;
+        }
+        std::fclose(f);
+        goal::LearnedIndicator learned(std::vector<float>(buf.begin(), buf.begin() + GOAL_MLP_PARAMS),
+                                       std::vector<float>(buf.begin() + GOAL_MLP_PARAMS, buf.end()), 0.25);
+        std::vector<double> le = learned.run(&mech, &adj, t_old, t_old - 1.0);
+        double ls = 0.0;
+        for (double e : le) ls += e;
+        std::printf(", \"learned_eta_sum\": %.17g, \"learned_fell_back\": %d", ls, learned.fell_back() ? 1 : 0);
+      }
+      if (extra == "adapt" && model != 2) {
+        // Solver::adapt: size field for 3x the elements, refine, solve again
+        const int64_t ne = disc->get_num_elems();
+        std::vector<double> size = goal::get_iso_target_size(disc.get(), eta, 3 * ne);
+        goal::Adapted next = goal::adapt(disc.get(), size);
+        goal::Disc* d2 = next.disc.get();
+        std::vector<double> c2((size_t)(3 * d2->get_num_nodes()));
+        std::vector<int32_t> t2((size_t)(4 * d2->get_num_elems()));
+        goal::check(goal_disc_get_mesh(d2->handle(), c2.data(), t2.data(), nullptr));
+        for (const char* face : {"zmin", "zmax"}) {
+          const double z = std::string(face) == "zmin" ? 0.0 : 1.0;
+          std::vector<int32_t> nodes;
+          for (int64_t k = 0; k < d2->get_num_nodes(); ++k)
+            if (c2[3 * k + 2] == z) nodes.push_back((int32_t)k);
+          goal::check(goal_disc_add_node_set(d2->handle(), face, (int64_t)nodes.size(), nodes.data()));
+        }
+        goal::Mechanics mech2(mp, d2);
+        goal::Primal primal2(pp, &mech2, solve);
+        primal2.build_data();
+        const int it2 = primal2.solve(t_old, t_old - 1.0);
+        goal::Functional f2("avg disp", -1, &primal2);
+        f2.compute(t_old, t_old - 1.0);
+        std::printf(", \"adapt\": {\"elems\": %lld, \"splits\": %lld, \"iters\": %d, \"J\": %.17g, \"norm\": %.6e}",
+                    (long long)d2->get_num_elems(), (long long)next.splits, it2, f2.get_value(), primal2.last_norm());
+      }
     }
     mech.update_states();
     std::printf("}\n");

@@ -18,11 +18,11 @@ pytestmark = pytest.mark.gpu
 EXE = Path(__file__).resolve().parent / "cpp" / "surface_test"
 
 
-def run_surface(order, quad, model, n, steps, tangent=64, solver="lu"):
+def run_surface(order, quad, model, n, steps, tangent=64, solver="lu", extra=None):
     if not EXE.exists():
         pytest.fail(f"{EXE} is not built: run __graft_entry__.build()")
-    r = subprocess.run([str(EXE), str(order), str(quad), str(model), str(n), str(steps), str(tangent), solver],
-                       capture_output=True, text=True, timeout=300)
+    args = [str(EXE), str(order), str(quad), str(model), str(n), str(steps), str(tangent), solver]
+    r = subprocess.run(args + ([extra] if extra else []), capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     return json.loads(r.stdout)
 
@@ -97,3 +97,62 @@ def test_cpp_surface_with_device_gmres(order, quad, model, n, steps):
     if order == 1:
         for k in ("eta_sum", "bound", "zR"):
             assert abs(gm[k] - lu[k]) <= 1e-7 * abs(lu[k]), k
+
+
+def test_cpp_surface_adapt_and_learned_indicator(tmp_path):
+    """The adapt step and the learned indicator through the C++ surface
+    (goal::get_iso_target_size, goal::adapt, goal::LearnedIndicator): the
+    adapted problem solves, and the learned indicators equal the Python
+    binding's on the same state."""
+    import numpy as np
+
+    import goal_amd as g
+    from goal_amd import learned as LP
+
+    with np.load(LP.DEFAULT_MODEL, allow_pickle=False) as f:
+        params, norms = f["params"].astype(np.float32), f["norms"].astype(np.float32)
+    net_file = tmp_path / "net.f32"
+    np.concatenate([params, norms]).tofile(net_file)
+    n = 4
+    out = subprocess.run([str(EXE), "1", "1", "1", str(n), "1", "64", "gmres", f"learned:{net_file}"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    cpp = json.loads(out.stdout)
+    ada = run_surface(1, 1, 1, n, 1, solver="gmres", extra="adapt")
+    print("learned", cpp.get("learned_eta_sum"), "nested", cpp["eta_sum"], "adapt", ada["adapt"], "J", ada["J"])
+    base = 6 * n ** 3
+    assert 1.5 * base < ada["adapt"]["elems"] < 4 * base
+    assert ada["adapt"]["norm"] < 1e-10
+    # the same state in Python: box, material, load step of the driver
+    disc = g.Disc.box(n, n, n, order=1, jitter=0.1, seed=11)
+    st = g.States(disc, 1, 1)
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=1, E=1000.0, nu=0.25, c0=1.0)], stabilization=True)
+    si = g.SolInfo(disc)
+    bcs = [(0, "zmin", "0.0"), (1, "zmin", "0.0"), (2, "zmin", "0.0"), (2, "zmax", "0.01*t")]
+    b, du = g.DeviceArray(disc.num_dofs), g.DeviceArray(disc.num_dofs)
+    for _ in range(10):
+        si.zero_all()
+        mech.jacobian(si, adjoint=False, save_states=True)
+        for comp, ns, expr in bcs:
+            g.dbc_apply(disc, si, comp, ns, expr, 1.0, True)
+        R, _, _ = si.dev()
+        g.check(g.lib.goal_device_scale(b.ptr, R, disc.num_dofs, -1.0))
+        du.fill(0.0)
+        si.solve(b, du, rtol=1e-12, max_iters=5000, restart=100)
+        disc.add_soln_dev(du)
+        si.zero_R()
+        mech.residual(si, save_states=True)
+        for comp, ns, expr in bcs:
+            g.dbc_apply(disc, si, comp, ns, expr, 1.0, False)
+        if si.norm_R() < 1e-10:
+            break
+    net = LP.load(LP.DEFAULT_MODEL)
+    feat = g.learn_features(mech, LP.QOI_X)
+    y = g.DeviceBuffer(disc.num_elems, np.float32)
+    outside = net.mlp.forward(feat, disc.num_elems, y)
+    py = float(np.sum(1000.0 * np.power(10.0, y.numpy().astype(np.float64))))
+    if cpp["learned_fell_back"]:
+        assert outside > 0.01 * disc.num_elems
+        assert abs(cpp["learned_eta_sum"] - cpp["eta_sum"]) <= 1e-9 * cpp["eta_sum"]
+    else:
+        assert abs(cpp["learned_eta_sum"] - py) <= 1e-5 * py
