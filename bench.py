@@ -424,17 +424,25 @@ def bench_learned(args) -> int:
     train_info = None
     if args.retrain or not path.exists():
         t0 = time.perf_counter()
-        if args.dataset:   # a dataset saved by an earlier run's checkpoint
-            with np.load(args.dataset, allow_pickle=False) as f:
-                X, Y, G = f["X"], f["Y"], f["G"]
-                runs = json.loads(bytes(f["runs"]).decode())
+        if args.dataset:   # datasets saved by earlier runs' checkpoints, concatenated
+            parts = []
+            for path in args.dataset.split(","):
+                with np.load(path, allow_pickle=False) as f:
+                    parts.append((f["X"], f["Y"], f["G"], json.loads(bytes(f["runs"]).decode())))
+            X = np.concatenate([q[0] for q in parts])
+            Y = np.concatenate([q[1] for q in parts])
+            offs = np.cumsum([0] + [len(q[3]) for q in parts])
+            G = np.concatenate([q[2] + o for q, o in zip(parts, offs)]).astype(np.int32)
+            runs = [r for q in parts for r in q[3]]
         else:
             ckpt = Path("gpurun_out") / "learned_dataset.npz" if Path("gpurun_out").is_dir() else None
-            X, Y, G, runs = LP.make_dataset(args.train_runs, seed=0, checkpoint=ckpt, log=log)
+            geos = tuple(args.train_geometries.split(","))
+            X, Y, G, runs = LP.make_dataset(args.train_runs, seed=args.train_seed, cube_cycles=args.train_cube_cycles,
+                                            checkpoint=ckpt, geometries=geos, log=log)
         data_s = time.perf_counter() - t0
         log(f"dataset: {X.shape[0]} samples from {args.train_runs} runs ({len(runs)} meshes) in {data_s:.1f} s")
         net, stats = LP.train(X, Y, epochs=40, batch=4096, lr=2e-3, seed=0, groups=G, log=log)
-        train_info = dict(stats, dataset_seconds=round(data_s, 1), runs=len({r["run"] for r in runs}),
+        train_info = dict(stats, dataset_seconds=round(data_s, 1), runs=len({(r.get("seed", 0), r["run"]) for r in runs}),
                           meshes=len(runs), samples=int(X.shape[0]),
                           max_train_elems={geo: max(r["elems"] for r in runs if r["geometry"] == geo)
                                            for geo in ("cube", "bar")})
@@ -542,7 +550,10 @@ def main() -> int:
     ap.add_argument("--save-model", default="", help="learned: where a retrained network is written")
     ap.add_argument("--train-runs", type=int, default=200)
     ap.add_argument("--train-only", action="store_true", help="learned: stop after training")
-    ap.add_argument("--dataset", default="", help="learned: train from a saved dataset (.npz) instead")
+    ap.add_argument("--dataset", default="", help="learned: train from saved datasets (.npz, comma-separated)")
+    ap.add_argument("--train-geometries", default="cube,bar", help="learned: problems of the training runs, in turn")
+    ap.add_argument("--train-cube-cycles", type=int, default=4, help="learned: adapt cycles of a cube training run")
+    ap.add_argument("--train-seed", type=int, default=0)
     ap.add_argument("--cycles", type=int, default=5)
     ap.add_argument("--final-elems", type=int, default=2_000_000, help="learned: final size of the cube runs")
     ap.add_argument("--final-elems-bar", type=int, default=200_000, help="learned: final size of the bar runs")

@@ -250,10 +250,10 @@ def adapt(m: Model, eta, target: int, split_ratio: float = 1.5, max_passes: int 
 # ------------------------------------------------------------------ training
 
 def make_dataset(num_runs: int, seed: int = 0, cube_cycles: int = 4, bar_cycles: int = 2, growth: float = 3.0,
-                 cap: int = 1200, lin_rtol: float = 1e-6, checkpoint=None, log=print):
+                 cap: int = 1200, lin_rtol: float = 1e-6, checkpoint=None, geometries=("cube", "bar"), log=print):
     """Training samples from nested DWR on seeded cube and bar problems:
     features X [N][12], targets log10(eta / E) [N] and the mesh index of each
-    sample.  Run r (cube for even r, bar for odd r) draws its problem from a
+    sample.  Run r (geometries[r % len(geometries)]) draws its problem from a
     generator seeded with (seed, r) and adapts `cube_cycles` / `bar_cycles`
     times with the nested indicators, so every mesh of the run contributes; at
     most `cap` random elements of a mesh are kept.  The linear solves stop at
@@ -268,19 +268,22 @@ def make_dataset(num_runs: int, seed: int = 0, cube_cycles: int = 4, bar_cycles:
     skipped = []
     for r in range(num_runs):
         rng = np.random.default_rng([seed, r])
-        geometry = "cube" if r % 2 == 0 else "bar"
+        geometry = geometries[r % len(geometries)]
         n = int(rng.choice([4, 8])) if geometry == "cube" else 4
         cycles = cube_cycles if geometry == "cube" else bar_cycles
         p = sample_problem(rng, geometry, n)
+        first = len(runs)
         try:
             _run_into(p, r, cycles, growth, cap, lin_rtol, rng, Xs, Ys, Gs, runs)
         except (g.GoalError, ValueError) as e:   # a failed solve ends the run; its earlier meshes stay
             skipped.append(r)
             if log:
                 log(f"[learned] run {r} stopped: {e}")
+        for rec in runs[first:]:
+            rec["seed"] = seed
         if (r + 1) % 10 == 0 and log:
             log(f"[learned] dataset: {r + 1}/{num_runs} runs, {sum(len(y) for y in Ys)} samples")
-        if checkpoint is not None and (r + 1) % 25 == 0:
+        if checkpoint is not None and ((r + 1) % 25 == 0 or r + 1 == num_runs):
             X_, Y_, G_, _ = arrays()
             np.savez(checkpoint, X=X_, Y=Y_, G=G_, runs=np.frombuffer(json.dumps(runs).encode(), dtype=np.uint8))
     return arrays()
