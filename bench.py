@@ -435,7 +435,7 @@ def bench_learned(args) -> int:
             G = np.concatenate([q[2] + o for q, o in zip(parts, offs)]).astype(np.int32)
             runs = [r for q in parts for r in q[3]]
         else:
-            ckpt = Path("gpurun_out") / "learned_dataset.npz" if Path("gpurun_out").is_dir() else None
+            ckpt = Path(args.dataset_out) if args.dataset_out else None
             geos = tuple(args.train_geometries.split(","))
             X, Y, G, runs = LP.make_dataset(args.train_runs, seed=args.train_seed, cube_cycles=args.train_cube_cycles,
                                             checkpoint=ckpt, geometries=geos, log=log)
@@ -445,8 +445,8 @@ def bench_learned(args) -> int:
         train_info = dict(stats, dataset_seconds=round(data_s, 1), runs=len({(r.get("seed", 0), r["run"]) for r in runs}),
                           meshes=len(runs), samples=int(X.shape[0]),
                           max_train_elems={geo: max(r["elems"] for r in runs if r["geometry"] == geo)
-                                           for geo in ("cube", "bar")})
-        out_path = Path(args.save_model) if args.save_model else Path("gpurun_out") / "learned_avgvm.npz"
+                                           for geo in ("cube", "bar") if any(r["geometry"] == geo for r in runs)})
+        out_path = Path(args.save_model) if args.save_model else Path("learned_avgvm.npz")
         out_path.parent.mkdir(parents=True, exist_ok=True)
         LP.save(net, out_path, meta=train_info)
         with open(out_path.with_suffix(".runs.json"), "w") as f:
@@ -554,6 +554,7 @@ def main() -> int:
     ap.add_argument("--train-geometries", default="cube,bar", help="learned: problems of the training runs, in turn")
     ap.add_argument("--train-cube-cycles", type=int, default=4, help="learned: adapt cycles of a cube training run")
     ap.add_argument("--train-seed", type=int, default=0)
+    ap.add_argument("--dataset-out", default="", help="learned: save the generated dataset here (every 25 runs)")
     ap.add_argument("--cycles", type=int, default=5)
     ap.add_argument("--final-elems", type=int, default=2_000_000, help="learned: final size of the cube runs")
     ap.add_argument("--final-elems-bar", type=int, default=200_000, help="learned: final size of the bar runs")

@@ -1051,8 +1051,8 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
     if (s->z.n < n) s->z.alloc(n);
     if (s->binv.n < 16 * nn) s->binv.alloc(16 * nn);
     const int nb = goal::multi_dot_blocks(n);
-    if (s->partial.n < (int64_t)(m + 1) * nb) s->partial.alloc((int64_t)(m + 1) * nb);
-    if (s->h1.n < m + 1) s->h1.alloc(m + 1);
+    if (s->partial.n < (int64_t)(m + 2) * nb) s->partial.alloc((int64_t)(m + 2) * nb);
+    if (s->h1.n < m + 2) s->h1.alloc(m + 2);
     if (s->h2.n < m + 1) s->h2.alloc(m + 1);
     if (s->y.n < m + 1) s->y.alloc(m + 1);
     if (s->solve_status.n < 1) s->solve_status.alloc(1);
@@ -1071,7 +1071,7 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
       sync();
       return std::sqrt(r);
     };
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hh1(m + 1), hh2(m + 1), yv(m + 1);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hh1(m + 2), hh2(m + 1), yv(m + 1);
     auto Hm = [&](int i, int j) -> double& { return H[(size_t)j * (m + 1) + i]; };
     int total = 0;
     double beta0 = -1.0, relres = 0.0;
@@ -1093,14 +1093,22 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
         double* w = V + (int64_t)(j + 1) * n;
         HIPCHK(goal::launch_block_apply(nn, s->binv.p, V + (int64_t)j * n, t, stream()));
         HIPCHK(goal::launch_spmv(n, rp, cols, A, t, w, stream()));
-        // classical Gram-Schmidt, twice (DGKS-style reorthogonalization)
-        HIPCHK(goal::launch_multi_dot(n, j + 1, V, n, w, s->partial.p, s->h1.p, stream()));
+        // classical Gram-Schmidt with the DGKS test (the reference's Belos
+        // orthogonalization, src/goal_linear_solve.cpp:53-90): a second pass
+        // only when the first removed more than 1 - 1/sqrt(2) of the norm
+        // w is V_{j+1}, so one more dot in the same pass gives |w|^2
+        HIPCHK(goal::launch_multi_dot(n, j + 2, V, n, w, s->partial.p, s->h1.p, stream()));
         HIPCHK(goal::launch_multi_axpy(n, j + 1, V, n, s->h1.p, -1.0, w, stream()));
-        HIPCHK(goal::launch_multi_dot(n, j + 1, V, n, w, s->partial.p, s->h2.p, stream()));
-        HIPCHK(goal::launch_multi_axpy(n, j + 1, V, n, s->h2.p, -1.0, w, stream()));
-        HIPCHK(hipMemcpyAsync(hh1.data(), s->h1.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, stream()));
-        HIPCHK(hipMemcpyAsync(hh2.data(), s->h2.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, stream()));
-        const double hn = norm2(w);
+        HIPCHK(hipMemcpyAsync(hh1.data(), s->h1.p, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream()));
+        double hn = norm2(w);  // waits, so hh1 has arrived
+        const double wn = std::sqrt(std::max(hh1[j + 1], 0.0));
+        std::fill(hh2.begin(), hh2.begin() + j + 1, 0.0);
+        if (hn < wn * 0.70710678118654752) {
+          HIPCHK(goal::launch_multi_dot(n, j + 1, V, n, w, s->partial.p, s->h2.p, stream()));
+          HIPCHK(goal::launch_multi_axpy(n, j + 1, V, n, s->h2.p, -1.0, w, stream()));
+          HIPCHK(hipMemcpyAsync(hh2.data(), s->h2.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, stream()));
+          hn = norm2(w);
+        }
         for (int i = 0; i <= j; ++i) Hm(i, j) = hh1[i] + hh2[i];
         Hm(j + 1, j) = hn;
         if (hn > 0.0) HIPCHK(goal::launch_scale(n, 1.0 / hn, w, w, stream()));

@@ -178,8 +178,9 @@ int goal_solinfo_dot_R(goal_solinfo s, const double* z_dev, double* dot);
  * goal::solve (src/goal_linear_solve.cpp:22-90: Belos block GMRES, block
  * size 1, "krylov size" = restart, DGKS orthogonalization, left MueLu
  * preconditioner).  Here: GMRES(restart) preconditioned on the right by the
- * inverse 4x4 diagonal block of every node, classical Gram-Schmidt applied
- * twice.  b, x are device vectors of num_dofs; x holds the initial guess on
+ * inverse 4x4 diagonal block of every node, classical Gram-Schmidt with the
+ * DGKS test (a second pass when the first removes more than 1 - 1/sqrt(2) of
+ * the norm).  b, x are device vectors of num_dofs; x holds the initial guess on
  * entry.  Stops when ||b - A x|| <= rtol ||b - A x0|| or after max_iters inner
  * iterations; like the reference it returns normally when the limit is hit
  * (*relres tells).  Outputs may be NULL. */
