@@ -438,7 +438,7 @@ def bench_learned(args) -> int:
             ckpt = Path(args.dataset_out) if args.dataset_out else None
             geos = tuple(args.train_geometries.split(","))
             X, Y, G, runs = LP.make_dataset(args.train_runs, seed=args.train_seed, cube_cycles=args.train_cube_cycles,
-                                            checkpoint=ckpt, geometries=geos, log=log)
+                                            bar_cycles=args.train_bar_cycles, checkpoint=ckpt, geometries=geos, log=log)
         data_s = time.perf_counter() - t0
         log(f"dataset: {X.shape[0]} samples from {args.train_runs} runs ({len(runs)} meshes) in {data_s:.1f} s")
         net, stats = LP.train(X, Y, epochs=40, batch=4096, lr=2e-3, seed=0, groups=G, log=log)
@@ -553,6 +553,7 @@ def main() -> int:
     ap.add_argument("--dataset", default="", help="learned: train from saved datasets (.npz, comma-separated)")
     ap.add_argument("--train-geometries", default="cube,bar", help="learned: problems of the training runs, in turn")
     ap.add_argument("--train-cube-cycles", type=int, default=4, help="learned: adapt cycles of a cube training run")
+    ap.add_argument("--train-bar-cycles", type=int, default=2, help="learned: adapt cycles of a bar training run")
     ap.add_argument("--train-seed", type=int, default=0)
     ap.add_argument("--dataset-out", default="", help="learned: save the generated dataset here (every 25 runs)")
     ap.add_argument("--cycles", type=int, default=5)

@@ -85,24 +85,39 @@ __global__ __launch_bounds__(kSolveThreads) void block_apply_kernel(int64_t nn, 
   for (int i = 0; i < 4; ++i) y[4 * a + i] = B[4 * i] * x0 + B[4 * i + 1] * x1 + B[4 * i + 2] * x2 + B[4 * i + 3] * x3;
 }
 
-// partial[j][block] = sum over the block's slice of V_j . w, j < k.
+// partial[j][block] = sum over the block's slice of V_j . w, j < k.  The
+// vectors go eight at a time: each row's w is read once per eight vectors and
+// the eight loads of a row are independent.
+constexpr int kDotGroup = 8;
+
 __global__ __launch_bounds__(kSolveThreads) void multi_dot_partial_kernel(int64_t n, int k, const double* V,
                                                                           int64_t ldv, const double* w,
                                                                           double* partial) {
-  __shared__ double red[kSolveThreads / 64];
+  __shared__ double red[kSolveThreads / 64][kDotGroup];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t stride = (int64_t)gridDim.x * kSolveThreads;
-  for (int j = 0; j < k; ++j) {
-    const double* v = V + (int64_t)j * ldv;
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kSolveThreads + tid; i < n; i += stride) s += v[i] * w[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) red[wv] = s;
+  for (int j0 = 0; j0 < k; j0 += kDotGroup) {
+    const int nj = min(kDotGroup, k - j0);
+    double s[kDotGroup];
+#pragma unroll
+    for (int jj = 0; jj < kDotGroup; ++jj) s[jj] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kSolveThreads + tid; i < n; i += stride) {
+      const double wi = w[i];
+#pragma unroll
+      for (int jj = 0; jj < kDotGroup; ++jj)
+        if (jj < nj) s[jj] += V[(int64_t)(j0 + jj) * ldv + i] * wi;
+    }
+#pragma unroll
+    for (int jj = 0; jj < kDotGroup; ++jj) {
+      double t = s[jj];
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+      if (lane == 0) red[wv][jj] = t;
+    }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < nj) {
       double t = 0.0;
-      for (int q = 0; q < kSolveThreads / 64; ++q) t += red[q];
-      partial[(int64_t)j * gridDim.x + blockIdx.x] = t;
+      for (int q = 0; q < kSolveThreads / 64; ++q) t += red[q][tid];
+      partial[(int64_t)(j0 + tid) * gridDim.x + blockIdx.x] = t;
     }
     __syncthreads();
   }
