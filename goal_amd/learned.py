@@ -32,7 +32,7 @@ import numpy as np
 
 import goal_amd as g
 
-# weights trained on the device from nested-DWR indicators of 200 seeded runs
+# weights trained on the device from nested-DWR indicators of 240 seeded runs
 DEFAULT_MODEL = Path(__file__).resolve().parent / "data" / "learned_avgvm.npz"
 QOI_X = 0.25
 QOI_SET = 1
