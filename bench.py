@@ -562,6 +562,10 @@ def main() -> int:
     ap.add_argument("--eval-seed", type=int, default=2024)
     ap.add_argument("--problems", default="cube,bar", help="learned: which evaluation problems to run")
     ap.add_argument("--blocks", type=int, default=0)
+    ap.add_argument("--overlap", type=int, default=-1,
+                    help="neo-Hookean: assemble in this many chunks, gathering finished chunks on a second stream "
+                         "(-1: the library's automatic choice, 0: off)")
+    ap.add_argument("--overlap-blocks", type=int, default=0, help="element workgroups per chunk (0: 5/6 resident)")
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=32, help="hexahedra per side of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -618,6 +622,9 @@ def main() -> int:
         mech.set_option("blocks", args.blocks)
     mech.set_option("xcd_remap", args.xcd_remap)
     mech.set_option("tangent", args.tangent)
+    if args.overlap >= 0:
+        mech.set_option("overlap", args.overlap)
+    mech.set_option("overlap_blocks", args.overlap_blocks)
     coords, _, _ = disc.mesh()
     disc.set_soln(smooth_state(coords) if wl["name"] == "cube" else bar_state(coords))
     si = g.SolInfo(disc)
@@ -657,6 +664,11 @@ def main() -> int:
     element_ms = statistics.mean(e for e, _ in splits)
     gather_ms = statistics.mean(gg for _, gg in splits)
     log(f"element pass {element_ms:.3f} ms, gather {gather_ms:.3f} ms")
+
+    def chunks(tangent):   # the library's choice for the "overlap" option
+        if args.overlap >= 0:
+            return args.overlap
+        return 32 if tangent == 32 and wl["model"] == 1 else 0
 
     # the other element-matrix precision, same mesh and iterate, for comparison
     other = 32 if args.tangent == 64 else 64
@@ -761,7 +773,12 @@ def main() -> int:
                              "HIP events around both; B_elem x elements / its duration",
             "traffic_source": traffic_src,
             "tangent_bits": args.tangent,
+            "chunks": chunks(args.tangent),
+            "chunked_split": "with chunks > 1 the element time runs from the first chunk to the last and "
+                             "overlaps the side-stream gathers of finished node runs; the gather time is "
+                             "the tail after the last chunk",
             f"tangent{other}": {"ms_per_step": round(other_ms, 4),
+                                "chunks": chunks(other),
                                 "elem_per_s": round(disc.num_elems / (other_ms * 1e-3), 1),
                                 "element_kernel_ms": round(statistics.mean(e for e, _ in osplit), 4),
                                 "gather_kernel_ms": round(statistics.mean(gg for _, gg in osplit), 4)},

@@ -236,12 +236,19 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
   auto& s = sm.b;
   const int tid = threadIdx.x;
   const int64_t ne = a.num_elems;
-  const int64_t ngroups = (ne + G - 1) / G;
+  int64_t g_lo = 0, ngroups = (ne + G - 1) / G;
+  if constexpr (MODEL == MODEL_NEOHOOKEAN) {
+    if (a.group_hi > 0) {
+      g_lo = a.group_lo;
+      ngroups = a.group_hi - a.group_lo;
+    }
+  }
   const int64_t nqtot = ne * NQ;
 
   for (int64_t it = 0;; ++it) {
-    const int64_t grp = group_of(it, ngroups, a.xcd_remap);
-    if (grp < 0) break;
+    const int64_t grp0 = group_of(it, ngroups, a.xcd_remap);
+    if (grp0 < 0) break;
+    const int64_t grp = g_lo + grp0;
     const int64_t e0 = grp * G;
     const int nv = (int)min((int64_t)G, ne - e0);  // elements of this group
 
@@ -811,6 +818,37 @@ hipError_t launch_gather(int order, const GatherArgs& g, int blocks, hipStream_t
   else
     hipLaunchKernelGGL((gather_kernel<10, double>), dim3(blocks), dim3(kGatherThreads), lds, s, g);
   return hipGetLastError();
+}
+
+int element_group_size(int order, int quad_order, int model) {
+  const int nq = quad_order == 1 ? 1 : 4;
+  if (order == 2) return nq == 1 ? Layout<10, 1, MODEL_NEOHOOKEAN>::G : Layout<10, 4, MODEL_NEOHOOKEAN>::G;
+  (void)model;  // G depends on the element and point counts only
+  return nq == 1 ? Layout<4, 1, MODEL_NEOHOOKEAN>::G : Layout<4, 4, MODEL_NEOHOOKEAN>::G;
+}
+
+namespace {
+template <int NN, int NQ, int MODEL>
+int resident_of_kind(int kind, bool f32, int64_t ne) {
+  const bool jac = kind == ASM_JACOBIAN || kind == ASM_ADJOINT;
+  if (jac && f32)
+    return kind == ASM_JACOBIAN ? resident_grid<NN, NQ, MODEL, ASM_JACOBIAN, float>(ne)
+                                : resident_grid<NN, NQ, MODEL, ASM_ADJOINT, float>(ne);
+  switch (kind) {
+    case ASM_RESIDUAL: return resident_grid<NN, NQ, MODEL, ASM_RESIDUAL, double>(ne);
+    case ASM_JACOBIAN: return resident_grid<NN, NQ, MODEL, ASM_JACOBIAN, double>(ne);
+    case ASM_ADJOINT: return resident_grid<NN, NQ, MODEL, ASM_ADJOINT, double>(ne);
+    default: return resident_grid<NN, NQ, MODEL, ASM_LOCALIZE, double>(ne);
+  }
+}
+}  // namespace
+
+int element_resident_grid(int kind, int order, int quad_order, int model, bool f32, int64_t ne) {
+  if (model != MODEL_NEOHOOKEAN) return 8;  // chunked assembly is neo-Hookean only
+  if (order == 2) return quad_order == 1 ? resident_of_kind<10, 1, MODEL_NEOHOOKEAN>(kind, f32, ne)
+                                         : resident_of_kind<10, 4, MODEL_NEOHOOKEAN>(kind, f32, ne);
+  return quad_order == 1 ? resident_of_kind<4, 1, MODEL_NEOHOOKEAN>(kind, f32, ne)
+                         : resident_of_kind<4, 4, MODEL_NEOHOOKEAN>(kind, f32, ne);
 }
 
 int gather_nodes_per_range() { return kGatherNodes; }

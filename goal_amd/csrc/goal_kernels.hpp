@@ -56,6 +56,9 @@ struct AsmArgs {
   int save_state;
   int xcd_remap;            // 1: give each XCD a contiguous slab of elements
   int* status;              // bit 0: J2 return map failed
+  // neo-Hookean only: this launch takes element groups [group_lo, group_hi)
+  // (group_hi == 0: all of them), so an assembly can be split into chunks
+  int64_t group_lo, group_hi;
 };
 
 // Node-owner gather of element residuals / matrices into R and the CSR values.
@@ -103,6 +106,10 @@ int gather_nodes_per_range();
 // Persistent grid for the gather: the workgroups resident at once, at most one per run.
 int gather_grid(int order, bool f32, int cap, int64_t num_ranges);
 hipError_t launch_qoi(int order, int quad_order, const QoiArgs& a, hipStream_t s);
+// Elements per workgroup iteration of the element kernel, and its persistent
+// grid for ne elements (the workgroups resident at once, at most one per group).
+int element_group_size(int order, int quad_order, int model);
+int element_resident_grid(int kind, int order, int quad_order, int model, bool f32, int64_t ne);
 
 // Utility kernels (goal_util.hip)
 // Device linear solve (goal_solve.hip).

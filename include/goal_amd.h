@@ -210,7 +210,13 @@ int goal_mech_destroy(goal_mech m);
  * FP64, so Newton still converges to the FP64 tolerance, in more iterations
  * at worst),
  * "timing" (1: record device events around the element and gather kernels of
- * each assembly, read back with goal_mech_timing). */
+ * each assembly, read back with goal_mech_timing),
+ * "overlap" (neo-Hookean only: > 1 runs the element pass in this many chunks
+ * and gathers each finished chunk's node ranges on a second stream, bitwise
+ * equal to one pass; 0 off; -1, the default, chunks 32-way for FP32-tangent
+ * Jacobians and not otherwise; not used under stream capture),
+ * "overlap_blocks" (element workgroups per chunk, 0 = 5/6 of the resident
+ * grid). */
 int goal_mech_set_option(goal_mech m, const char* key, int64_t value);
 /* Device time of the last assembly's element pass and node gather, in ms
  * (waits for it).  Needs the "timing" option.  No counterpart in the
