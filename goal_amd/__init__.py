@@ -143,6 +143,10 @@ _sig("goal_asm_localize", _i32, _p, _p, _p, _p)
 _sig("goal_eval", _i32, C.c_char_p, _d, _d, _d, _d, _dp)
 _sig("goal_dbc_apply", _i32, _p, _p, _i32, C.c_char_p, C.c_char_p, _d, _i32)
 _sig("goal_dbc_apply_values", _i32, _p, _p, _i32, _i64, _p, _p, _i32)
+_sig("goal_tbc_apply", _i32, _p, _p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, _d, _p)
+_sig("goal_disc_add_side_set", _i32, _p, C.c_char_p, _i64, _p, _p)
+_sig("goal_disc_side_set_size", _i32, _p, C.c_char_p, _i64p)
+_sig("goal_disc_get_side_set", _i32, _p, C.c_char_p, _p, _p)
 _sig("goal_batch_create", _i32, _i32, _p, _p, C.POINTER(_p))
 _sig("goal_batch_destroy", _i32, _p)
 _sig("goal_batch_tune", _i32, _p, _i32, _p, _p)
@@ -497,6 +501,24 @@ class Disc:
         nodes = np.ascontiguousarray(nodes, dtype=np.int32)
         check(lib.goal_disc_add_node_set(self.h, name.encode(), nodes.size, _ptr(nodes)))
 
+    def side_set(self, name: str):
+        """(elements, local faces) of a side set; a node set's name gives its
+        boundary faces."""
+        n = C.c_int64(0)
+        check(lib.goal_disc_side_set_size(self.h, name.encode(), C.byref(n)))
+        elems = np.empty(n.value, dtype=np.int32)
+        faces = np.empty(n.value, dtype=np.int32)
+        if n.value:
+            check(lib.goal_disc_get_side_set(self.h, name.encode(), _ptr(elems), _ptr(faces)))
+        return elems, faces
+
+    def add_side_set(self, name: str, elems, faces) -> None:
+        elems = np.ascontiguousarray(elems, dtype=np.int32)
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        if elems.shape != faces.shape:
+            raise ValueError("one local face per element")
+        check(lib.goal_disc_add_side_set(self.h, name.encode(), elems.size, _ptr(elems), _ptr(faces)))
+
     def set_elem_sets(self, num_sets: int, elem_set) -> None:
         es = np.ascontiguousarray(elem_set, dtype=np.int32)
         check(lib.goal_disc_set_elem_sets(self.h, num_sets, _ptr(es)))
@@ -719,6 +741,14 @@ def dbc_apply_values(disc: Disc, s: SolInfo, comp: int, nodes, values, jacobian:
     nodes = np.ascontiguousarray(nodes, dtype=np.int32)
     values = np.ascontiguousarray(values, dtype=np.float64)
     check(lib.goal_dbc_apply_values(disc.h, s.h, comp, nodes.size, _ptr(nodes), _ptr(values), int(bool(jacobian))))
+
+
+def tbc_apply(disc: Disc, s: SolInfo, side_set: str, exprs, t: float, weight=None) -> None:
+    """One traction entry [side set, T_x, T_y, T_z] (src/goal_tbcs.cpp:30-66);
+    weight: None (weight N_a) or a device DOF vector z (weight z_i N_a)."""
+    tx, ty, tz = (str(e).encode() for e in exprs)
+    w = None if weight is None else (weight.ptr if hasattr(weight, "ptr") else _p(int(weight)))
+    check(lib.goal_tbc_apply(disc.h, s.h, side_set.encode(), tx, ty, tz, t, w))
 
 
 class Nested:

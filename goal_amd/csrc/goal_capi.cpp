@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -190,6 +191,11 @@ struct goal_disc_s {
   DevBuf<int64_t> row_ptr, inc_ptr;
   std::map<std::string, std::unique_ptr<DevBuf<int32_t>>> dev_node_sets;
   DevBuf<double> dbc_values;
+  // side sets as 4 * element + local face: given ones, and ones derived from
+  // the node set of the same name (dropped when that node set changes)
+  std::map<std::string, std::vector<int32_t>> side_sets, derived_side_sets;
+  DevBuf<int64_t> tbc_rows;
+  DevBuf<double> tbc_values;
 };
 
 struct goal_states_s {
@@ -893,6 +899,7 @@ int goal_disc_add_node_set(goal_disc d, const char* name, int64_t n, const int32
     for (int32_t x : v) require(x >= 0 && x < d->mesh.num_nodes, "node set entry out of range");
     d->mesh.node_sets[name] = std::move(v);
     d->dev_node_sets.erase(name);
+    d->derived_side_sets.erase(name);
     return GOAL_OK;
   });
 }
@@ -913,6 +920,91 @@ int goal_disc_get_node_set(goal_disc d, const char* name, int32_t* nodes) {
     auto it = d->mesh.node_sets.find(name);
     if (it == d->mesh.node_sets.end()) return set_error(GOAL_ERR_INVALID, "unknown node set: %s", name);
     std::copy(it->second.begin(), it->second.end(), nodes);
+    return GOAL_OK;
+  });
+}
+
+// ------------------------------------------------------------ side sets
+namespace {
+
+// Local vertices of tet face f, in apf's order.
+constexpr int kTetFace[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+
+int tet_edge_between(int a, int b) {
+  for (int k = 0; k < 6; ++k)
+    if ((goal::kTetEdge[k][0] == a && goal::kTetEdge[k][1] == b) ||
+        (goal::kTetEdge[k][0] == b && goal::kTetEdge[k][1] == a))
+      return k;
+  return -1;
+}
+
+// The side set `name`: a given one, else the faces on the mesh boundary whose
+// three vertices all lie in the node set of that name (for a box face node
+// set: that face's triangles, the side set goal's assoc file names for it).
+const std::vector<int32_t>& side_set(goal_disc d, const char* name) {
+  auto given = d->side_sets.find(name);
+  if (given != d->side_sets.end()) return given->second;
+  auto derived = d->derived_side_sets.find(name);
+  if (derived != d->derived_side_sets.end()) return derived->second;
+  auto ns = d->mesh.node_sets.find(name);
+  if (ns == d->mesh.node_sets.end())
+    throw std::invalid_argument(std::string("unknown side set (and no node set of that name): ") + name);
+  std::vector<char> in(d->mesh.num_nodes, 0);
+  for (int32_t n : ns->second) in[n] = 1;
+  const int npe = d->mesh.nodes_per_elem;
+  std::map<std::array<int32_t, 3>, std::pair<int32_t, int>> seen;  // sorted vertices -> (side, count)
+  for (int64_t e = 0; e < d->mesh.num_elems; ++e) {
+    const int32_t* c = &d->mesh.conn[e * npe];
+    for (int f = 0; f < 4; ++f) {
+      std::array<int32_t, 3> v = {c[kTetFace[f][0]], c[kTetFace[f][1]], c[kTetFace[f][2]]};
+      if (!in[v[0]] || !in[v[1]] || !in[v[2]]) continue;
+      std::sort(v.begin(), v.end());
+      auto it = seen.find(v);
+      if (it == seen.end())
+        seen.emplace(v, std::make_pair((int32_t)(4 * e + f), 1));
+      else
+        ++it->second.second;  // shared by two elements: an interior face
+    }
+  }
+  std::vector<int32_t> sides;
+  for (auto const& kv : seen)
+    if (kv.second.second == 1) sides.push_back(kv.second.first);
+  std::sort(sides.begin(), sides.end());
+  return d->derived_side_sets[name] = std::move(sides);
+}
+
+}  // namespace
+
+int goal_disc_add_side_set(goal_disc d, const char* name, int64_t n, const int32_t* elems, const int32_t* faces) {
+  return guard([&]() -> int {
+    require(d && name && n >= 0 && (n == 0 || (elems && faces)), "goal_disc_add_side_set: bad arguments");
+    std::vector<int32_t> v((size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+      require(elems[k] >= 0 && elems[k] < d->mesh.num_elems, "side set element out of range");
+      require(faces[k] >= 0 && faces[k] < 4, "side set local face must be 0..3");
+      v[k] = 4 * elems[k] + faces[k];
+    }
+    d->side_sets[name] = std::move(v);
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_side_set_size(goal_disc d, const char* name, int64_t* n) {
+  return guard([&]() -> int {
+    require(d && name && n, "goal_disc_side_set_size: bad arguments");
+    *n = (int64_t)side_set(d, name).size();
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_get_side_set(goal_disc d, const char* name, int32_t* elems, int32_t* faces) {
+  return guard([&]() -> int {
+    require(d && name && elems && faces, "goal_disc_get_side_set: bad arguments");
+    const std::vector<int32_t>& v = side_set(d, name);
+    for (size_t k = 0; k < v.size(); ++k) {
+      elems[k] = v[k] / 4;
+      faces[k] = v[k] % 4;
+    }
     return GOAL_OK;
   });
 }
@@ -1812,6 +1904,95 @@ int goal_dbc_apply_values(goal_disc d, goal_solinfo s, int comp, int64_t n, cons
     dn.upload(nodes, n);
     std::vector<double> v(values, values + n);
     return dbc_apply_dev(d, s, comp, dn.p, n, v, jacobian);
+  });
+}
+
+// One traction entry (src/goal_tbcs.cpp:30-66): per face of the side set, the
+// reference's one-point rule -- the centroid, weight 1/2, dv = twice the area
+// (apf getIntPoint / getIntWeight / getDV on a triangle) -- so each face node
+// a takes R[a, i] -= T_i(x_c, t) W_{a,i} |face|.  The weight is the "uw"
+// evaluator's value at the centroid: N_a (VectorWeight) when weight_dev is
+// null, z_i(x_c) N_a (DisplacementAdjoint over the DOF vector weight_dev,
+// src/goal_displacement_adjoint.cpp:37-53) otherwise.  N_a(centroid) is 1/3
+// on a linear face, -1/9 (vertices) and 4/9 (edge nodes) on a quadratic one.
+// Shares of a row are summed on the host in side-set order, then one kernel
+// subtracts them.  A dead load: the Jacobian is untouched.
+int goal_tbc_apply(goal_disc d, goal_solinfo s, const char* side_set_name, const char* tx, const char* ty,
+                   const char* tz, double t, const double* weight_dev) {
+  return guard([&]() -> int {
+    require(d && s && s->d == d && side_set_name && tx && ty && tz, "goal_tbc_apply: bad arguments");
+    const std::vector<int32_t>& sides = side_set(d, side_set_name);
+    const goal::Expr T[3] = {goal::Expr(tx), goal::Expr(ty), goal::Expr(tz)};
+    const int npe = d->mesh.nodes_per_elem;
+    const bool p2 = d->mesh.order == 2;
+    const int nfn = p2 ? 6 : 3;
+    struct Face {
+      int32_t nodes[6];
+      double N[6], T[3], w_dv;
+    };
+    std::vector<Face> faces(sides.size());
+    for (size_t k = 0; k < sides.size(); ++k) {
+      Face& F = faces[k];
+      const int64_t e = sides[k] / 4;
+      const int f = sides[k] % 4;
+      const int32_t* c = &d->mesh.conn[e * npe];
+      const int lv[3] = {kTetFace[f][0], kTetFace[f][1], kTetFace[f][2]};
+      const double* X[3];
+      for (int j = 0; j < 3; ++j) X[j] = &d->mesh.coords[3 * (int64_t)c[lv[j]]];
+      double a[3], b[3], xc[3];
+      for (int i = 0; i < 3; ++i) {
+        a[i] = X[1][i] - X[0][i];
+        b[i] = X[2][i] - X[0][i];
+        xc[i] = (X[0][i] + X[1][i] + X[2][i]) / 3.0;
+      }
+      const double nrm[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+      F.w_dv = 0.5 * std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+      for (int i = 0; i < 3; ++i) F.T[i] = T[i](xc[0], xc[1], xc[2], t);
+      for (int j = 0; j < 3; ++j) {
+        F.nodes[j] = c[lv[j]];
+        F.N[j] = p2 ? -1.0 / 9.0 : 1.0 / 3.0;
+      }
+      if (p2)
+        for (int j = 0; j < 3; ++j) {
+          F.nodes[3 + j] = c[4 + tet_edge_between(lv[j], lv[(j + 1) % 3])];
+          F.N[3 + j] = 4.0 / 9.0;
+        }
+    }
+    // the displacement rows of every face node: where the loads go, and
+    // where the weight field is read
+    std::vector<int64_t> rows;
+    for (const Face& F : faces)
+      for (int j = 0; j < nfn; ++j)
+        for (int i = 0; i < goal::kDim; ++i) rows.push_back(goal::kNumEqs * (int64_t)F.nodes[j] + i);
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    const int64_t nr = (int64_t)rows.size();
+    auto index = [&](int64_t row) { return std::lower_bound(rows.begin(), rows.end(), row) - rows.begin(); };
+    materialize(s);
+    d->tbc_rows.upload(rows.data(), nr);
+    std::vector<double> z;
+    if (weight_dev) {
+      d->tbc_values.alloc(nr);
+      HIPCHK(goal::launch_take_rows(d->tbc_rows.p, weight_dev, nr, d->tbc_values.p, stream()));
+      z.resize(nr);
+      d->tbc_values.download(z.data(), nr);
+    }
+    std::vector<double> load(nr, 0.0);
+    for (const Face& F : faces) {
+      double zc[3] = {1.0, 1.0, 1.0};
+      if (weight_dev)
+        for (int i = 0; i < goal::kDim; ++i) {
+          zc[i] = 0.0;
+          for (int j = 0; j < nfn; ++j) zc[i] += z[index(goal::kNumEqs * (int64_t)F.nodes[j] + i)] * F.N[j];
+        }
+      for (int j = 0; j < nfn; ++j)
+        for (int i = 0; i < goal::kDim; ++i)
+          load[index(goal::kNumEqs * (int64_t)F.nodes[j] + i)] += F.T[i] * zc[i] * F.N[j] * F.w_dv;
+    }
+    d->tbc_values.upload(load.data(), nr);
+    HIPCHK(goal::launch_sub_rows(d->tbc_rows.p, d->tbc_values.p, nr, s->R.p, stream()));
+    sync();
+    return GOAL_OK;
   });
 }
 

@@ -141,6 +141,10 @@ hipError_t launch_reduce(int op, const double* x, const double* y, int64_t n, do
 hipError_t launch_dbc(const int32_t* nodes, const double* values, int64_t n, int comp, const double* x,
                       double* R, double* dMdu, const int64_t* row_ptr, const int32_t* cols,
                       double* vals, hipStream_t s);
+// Traction loads (src/goal_tbcs.cpp:30-66): R[rows[k]] -= values[k], rows distinct
+hipError_t launch_sub_rows(const int64_t* rows, const double* values, int64_t n, double* R, hipStream_t s);
+// out[k] = src[rows[k]]
+hipError_t launch_take_rows(const int64_t* rows, const double* src, int64_t n, double* out, hipStream_t s);
 
 // DWR (goal_dwr.hip)
 hipError_t launch_set_coarse(double* z, const int32_t* edge_parents, int64_t num_old, int64_t num_new,

@@ -1,5 +1,6 @@
 // Streaming helper kernels: fills, copies, axpy, deterministic reductions,
-// CSR expansion from the node adjacency, and Dirichlet row fix-up.
+// CSR expansion from the node adjacency, Dirichlet row fix-up and boundary
+// loads.
 #include "goal_kernels.hpp"
 
 namespace goal {
@@ -95,7 +96,33 @@ __global__ __launch_bounds__(kBlock) void dbc_kernel(const int32_t* nodes, const
   }
 }
 
+// Boundary loads (src/goal_tbcs.cpp:30-66): R[rows[k]] -= values[k].  The
+// host sums every face's share of a row first, so rows are distinct.
+__global__ __launch_bounds__(kBlock) void sub_rows_kernel(const int64_t* rows, const double* values, int64_t n,
+                                                          double* R) {
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+    R[rows[k]] -= values[k];
+}
+
+__global__ __launch_bounds__(kBlock) void take_rows_kernel(const int64_t* rows, const double* src, int64_t n,
+                                                           double* out) {
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+    out[k] = src[rows[k]];
+}
+
 }  // namespace
+
+hipError_t launch_take_rows(const int64_t* rows, const double* src, int64_t n, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(take_rows_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, rows, src, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sub_rows(const int64_t* rows, const double* values, int64_t n, double* R, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sub_rows_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, rows, values, n, R);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -132,6 +132,15 @@ int goal_disc_get_lids(goal_disc d, int32_t* lids);
 int goal_disc_add_node_set(goal_disc d, const char* name, int64_t n, const int32_t* nodes);
 int goal_disc_node_set_size(goal_disc d, const char* name, int64_t* n);
 int goal_disc_get_node_set(goal_disc d, const char* name, int32_t* nodes);
+/* Side sets (Disc::get_sides, src/goal_disc.cpp:174-177): element faces as
+ * (element, local face) pairs, local face f of a tet being the vertices
+ * {0,1,2}, {0,1,3}, {1,2,3}, {0,2,3} for f = 0..3 (apf's order).  A name with
+ * no side set given but a node set of that name stands for the boundary faces
+ * whose three vertices all lie in that node set (a box face's triangles for
+ * the box meshes' face node sets), in ascending 4 * element + face order. */
+int goal_disc_add_side_set(goal_disc d, const char* name, int64_t n, const int32_t* elems, const int32_t* faces);
+int goal_disc_side_set_size(goal_disc d, const char* name, int64_t* n);
+int goal_disc_get_side_set(goal_disc d, const char* name, int32_t* elems, int32_t* faces);
 /* Element sets (Disc::get_elems / compute_elem_sets, src/goal_disc.cpp:169-172, 347-359) */
 int goal_disc_set_elem_sets(goal_disc d, int num_sets, const int32_t* elem_set);
 /* The u,p fields ("u", "p" created in src/goal_mechanics.cpp:71-85): x[4*node+eq]. */
@@ -246,7 +255,7 @@ int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* va
  * z_diff*N_a (uw, pw) and z_coarse*N_a (pwc), R += r_e. */
 int goal_asm_localize(goal_mech m, goal_solinfo s, const double* z_diff_dev, const double* z_coarse_dev);
 
-/* ------------------------------------------------------------ Dirichlet */
+/* -------------------------------------------------- Dirichlet and traction */
 /* goal::eval (src/goal_control.cpp:107-123) */
 int goal_eval(const char* expr, double x, double y, double z, double t, double* out);
 /* One entry [comp, node set, expr] of set_resid_dbcs (jacobian = 0) or
@@ -256,6 +265,18 @@ int goal_dbc_apply(goal_disc d, goal_solinfo s, int comp, const char* node_set, 
 /* Same with explicit node ids and prescribed values. */
 int goal_dbc_apply_values(goal_disc d, goal_solinfo s, int comp, int64_t n, const int32_t* nodes,
                           const double* values, int jacobian);
+/* One entry [side set, T_x, T_y, T_z] of set_tbcs (src/goal_tbcs.cpp:30-66):
+ * a dead traction load.  Per face, the reference's one-point rule (centroid,
+ * w dv = face area): R[a, i] -= T_i(x_c, t) W_{a,i} |face| for every face node
+ * a, with the "uw" weight W_{a,i} = N_a(centroid) when weight_dev is NULL
+ * (primal and adjoint chains) or z_i(x_c) N_a(centroid) for a device DOF
+ * vector z = weight_dev (the error chain's DisplacementAdjoint,
+ * src/goal_displacement_adjoint.cpp:37-53).  Residual only; call it after the
+ * assembly and before the Dirichlet rows, as Primal::compute_resid /
+ * compute_jacob and NestedAdjoint::compute_adjoint / localize do
+ * (src/goal_primal.cpp:75-109, src/goal_nested_adjoint.cpp:163-234). */
+int goal_tbc_apply(goal_disc d, goal_solinfo s, const char* side_set, const char* tx, const char* ty,
+                   const char* tz, double t, const double* weight_dev);
 
 /* --------------------------------------------------------- batched assembly */
 /* Many meshes assembled as one launch: the Jacobian-mode assemblies of n

@@ -15,7 +15,7 @@
 //   * FADT is a tag type: Jacobian chains use the hand-derived analytic
 //     tangent, not dual numbers.
 //   * ParameterList input is replaced by plain structs (MechanicsParams,
-//     PrimalParams, DirichletBC).
+//     PrimalParams, DirichletBC, TractionBC).
 #pragma once
 
 #include <cmath>
@@ -396,6 +396,22 @@ inline void set_jac_dbcs(std::vector<DirichletBC> const& bcs, SolInfo* s, double
     check(goal_dbc_apply(s->get_disc()->handle(), s->handle(), b.comp, b.set.c_str(), b.value.c_str(), t, 1));
 }
 
+// ------------------------------------------------------------ tractions
+// One entry [side set, T_x, T_y, T_z] of the "traction bcs" list
+// (src/goal_tbcs.cpp:30-66).  A side set name with no side set given to the
+// disc stands for the boundary faces of the node set of that name.
+struct TractionBC {
+  std::string set;
+  std::string value[3];
+};
+// set_tbcs (src/goal_tbcs.cpp:68-79).  weight_dev: null for the primal and
+// adjoint chains (weight N_a), the z_diff DOF vector for the error chain.
+inline void set_tbcs(std::vector<TractionBC> const& bcs, SolInfo* s, double t, const double* weight_dev = nullptr) {
+  for (auto const& b : bcs)
+    check(goal_tbc_apply(s->get_disc()->handle(), s->handle(), b.set.c_str(), b.value[0].c_str(),
+                         b.value[1].c_str(), b.value[2].c_str(), t, weight_dev));
+}
+
 // Linear solve callback standing in for goal::solve (Belos GMRES + MueLu,
 // src/goal_linear_solve.cpp:72-90): solve A x = b with A the device CSR of
 // s (row_ptr / cols from the disc, values from s), b and x device vectors.
@@ -420,6 +436,7 @@ struct GmresSolve {
 
 struct PrimalParams {
   std::vector<DirichletBC> dirichlet;
+  std::vector<TractionBC> traction;  // "traction bcs"
   int max_iters = 5;       // "nonlinear max iters"
   double tolerance = 1e-8; // "nonlinear tolerance"
 };
@@ -448,6 +465,7 @@ class Primal {
     sol_info_->zero_R();
     set_time(residual_, t_now, t_old);
     assemble(residual_, sol_info_.get());
+    set_tbcs(params_.traction, sol_info_.get(), t_now);
     sol_info_->gather_R();
     set_resid_dbcs(params_.dirichlet, sol_info_.get(), t_now);
   }
@@ -456,6 +474,7 @@ class Primal {
     sol_info_->zero_all();
     set_time(jacobian_, t_now, t_old);
     assemble(jacobian_, sol_info_.get());
+    set_tbcs(params_.traction, sol_info_.get(), t_now);
     sol_info_->gather_all();
     set_jac_dbcs(params_.dirichlet, sol_info_.get(), t_now);
     sol_info_->complete_fill();
@@ -581,9 +600,9 @@ class Nested {
 class NestedAdjoint {
  public:
   NestedAdjoint(Mechanics* base_mech, std::string const& qoi, int qoi_set, std::vector<DirichletBC> bcs,
-                LinearSolve solve)
+                LinearSolve solve, std::vector<TractionBC> tbcs = {})
       : mp_(base_mech->params()), base_mech_(base_mech), qoi_(qoi), qoi_set_(qoi_set), bcs_(std::move(bcs)),
-        base_(base_mech->get_disc()), solve_(std::move(solve)) {}
+        tbcs_(std::move(tbcs)), base_(base_mech->get_disc()), solve_(std::move(solve)) {}
 
   std::vector<double> run(double t_now, double t_old) {
     Nested nested(base_);
@@ -614,6 +633,7 @@ class NestedAdjoint {
       si.zero_all();
       set_time(adj, t_now, t_old);
       assemble(adj, &si);
+      set_tbcs(tbcs_, &si, t_now);
       set_jac_dbcs(bcs_, &si, t_now);
       double *R = nullptr, *dMdu = nullptr;
       si.dev(&R, &dMdu, nullptr);
@@ -633,6 +653,7 @@ class NestedAdjoint {
       si.zero_R();
       set_time(err, t_now, t_old);
       assemble(err, &si);
+      set_tbcs(tbcs_, &si, t_now, zd);
       set_resid_dbcs(bcs_, &si, t_now);
       si.dev(&R, nullptr, nullptr);
       bound_ = sum_contribs(fine, R);
@@ -656,6 +677,7 @@ class NestedAdjoint {
   std::string qoi_;
   int qoi_set_;
   std::vector<DirichletBC> bcs_;
+  std::vector<TractionBC> tbcs_;
   Disc* base_;
   LinearSolve solve_;
   double bound_ = 0.0, zR_ = 0.0;

@@ -47,6 +47,7 @@ _sig("oracle_assemble", _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i64, _i64, _p
 _sig("oracle_qoi", _i32, _i32, _i32, _i32, _i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _i32,
      C.POINTER(C.c_double), _p, _p)
 _sig("oracle_dbc", None, _i64, _p, _p, _i32, _p, _p, _p, _p, _p, _p)
+_sig("oracle_traction", None, _i32, _i64, _p, _p, _p, _p, _p)
 _sig("oracle_set_coarse", None, _p, _p, _i64, _i64)
 _sig("oracle_compute_error", None, _i32, _i64, _p, _p, _p)
 _sig("oracle_sum_contribs", C.c_double, _i64, _p, _p)
@@ -155,6 +156,54 @@ def dbc(mesh: Mesh, nodes, values, comp, x, R, dMdu=None, vals=None):
     values = _f64(values)
     lib.oracle_dbc(nodes.size, _ptr(nodes), _ptr(values), comp, _ptr(_f64(x)), _ptr(R), _ptr(dMdu),
                    _ptr(mesh.row_ptr), _ptr(mesh.cols), _ptr(vals))
+
+
+# local vertices of tet face f (apf's order) and the tet edges in the P2 node
+# order {01, 12, 20, 03, 13, 23}
+_TET_FACES = ((0, 1, 2), (0, 1, 3), (1, 2, 3), (0, 2, 3))
+_TET_EDGES = ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))
+
+
+def boundary_faces(mesh: Mesh, nodes):
+    """Faces on the mesh boundary with all three vertices in `nodes`, as node
+    lists (3 vertices, then for P2 the nodes of edges v0v1, v1v2, v2v0), in
+    ascending (element, local face) order: the side set of a box face."""
+    inset = np.zeros(mesh.num_nodes, dtype=bool)
+    inset[np.asarray(nodes)] = True
+    conn = mesh.conn.reshape(mesh.num_elems, -1)
+    cand = []
+    for f, (a, b, c) in enumerate(_TET_FACES):
+        ok = np.nonzero(inset[conn[:, a]] & inset[conn[:, b]] & inset[conn[:, c]])[0]
+        cand += [(int(e), f) for e in ok]
+    keys = [tuple(sorted(conn[e, list(_TET_FACES[f])])) for e, f in cand]
+    count = {}
+    for k in keys:
+        count[k] = count.get(k, 0) + 1
+    out = []
+    for (e, f), k in sorted(zip(cand, keys)):
+        if count[k] != 1:
+            continue
+        lv = _TET_FACES[f]
+        row = [int(conn[e, v]) for v in lv]
+        if mesh.order == 2:
+            for j in range(3):
+                pair = {lv[j], lv[(j + 1) % 3]}
+                row.append(int(conn[e, 4 + next(i for i, ed in enumerate(_TET_EDGES) if set(ed) == pair)]))
+        out.append(row)
+    return np.asarray(out, dtype=np.int32).reshape(-1, 3 if mesh.order == 1 else 6)
+
+
+def traction(mesh: Mesh, faces, T, t, R, z=None):
+    """One traction entry (src/goal_tbcs.cpp:30-66): T(x, y, z, t) -> 3
+    components, evaluated at each face centroid; R -= T W |face| per face node
+    (W = N_a, or z_i N_a with a DOF vector z)."""
+    faces = _i32a(faces)
+    nfn = 3 if mesh.order == 1 else 6
+    c = mesh.coords.reshape(-1, 3)
+    xc = c[faces.reshape(-1, nfn)[:, :3]].mean(axis=1)
+    Tv = np.ascontiguousarray([T(x[0], x[1], x[2], t) for x in xc], dtype=np.float64).reshape(-1, 3)
+    lib.oracle_traction(mesh.order, faces.size // nfn, _ptr(faces), _ptr(mesh.coords), _ptr(Tv),
+                        _ptr(_f64(z)), _ptr(R))
 
 
 def set_coarse(z, edge_parents, num_old):

@@ -841,6 +841,46 @@ void oracle_dbc(int64_t n, const int32_t* nodes, const double* values, int comp,
   }
 }
 
+// set_tbcs, one entry (src/goal_tbcs.cpp:30-66).  Faces come as node lists:
+// 3 vertices, then (quadratic) the nodes of edges v0v1, v1v2, v2v0.  The
+// reference integrates each face with apf's one-point triangle rule: the
+// centroid xi = (1/3, 1/3), weight 1/2, dv = |dx/dxi| = twice the area.  T is
+// the traction at the centroid per face; z (optional DOF vector) makes the
+// weight z_i(centroid) N_a (DisplacementAdjoint) instead of N_a.
+void oracle_traction(int order, int64_t nf, const int32_t* fnodes, const double* coords, const double* T,
+                     const double* z, double* R) {
+  const int nfn = order == 1 ? 3 : 6;
+  // triangle shape functions at the centroid, from the barycentric
+  // coordinates l = (1/3, 1/3, 1/3): linear N = l_k; quadratic corner
+  // l_k (2 l_k - 1), edge 4 l_j l_k
+  const double l = 1.0 / 3.0;
+  double N[6];
+  for (int k = 0; k < 3; ++k) N[k] = order == 1 ? l : l * (2.0 * l - 1.0);
+  for (int k = 3; k < 6; ++k) N[k] = 4.0 * l * l;
+  const double ipw = 0.5;
+  for (int64_t f = 0; f < nf; ++f) {
+    const int32_t* fn = &fnodes[f * nfn];
+    const double* X0 = &coords[3 * (int64_t)fn[0]];
+    const double* X1 = &coords[3 * (int64_t)fn[1]];
+    const double* X2 = &coords[3 * (int64_t)fn[2]];
+    double a[3], b[3];
+    for (int i = 0; i < 3; ++i) {
+      a[i] = X1[i] - X0[i];
+      b[i] = X2[i] - X0[i];
+    }
+    const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+    const double dv = std::sqrt(cx * cx + cy * cy + cz * cz);
+    double zc[3] = {1.0, 1.0, 1.0};
+    if (z)
+      for (int i = 0; i < 3; ++i) {
+        zc[i] = 0.0;
+        for (int k = 0; k < nfn; ++k) zc[i] += z[4 * (int64_t)fn[k] + i] * N[k];
+      }
+    for (int k = 0; k < nfn; ++k)
+      for (int i = 0; i < 3; ++i) R[4 * (int64_t)fn[k] + i] -= T[3 * f + i] * zc[i] * N[k] * ipw * dv;
+  }
+}
+
 // Nested::set_coarse (in place): new vertex = mean of its two parents.
 void oracle_set_coarse(double* z, const int32_t* edge_parents, int64_t num_old, int64_t num_new) {
   for (int64_t k = 0; k < num_new; ++k) {

@@ -5,12 +5,19 @@
 // meshes this program is run on.  Prints one JSON object.
 //
 // usage: surface_test <order> <quad_order> <model> <n> <steps> [tangent bits: 64 | 32] [solver: lu | gmres]
-//                     [extra: adapt | learned:<file>]
+//                     [extra: adapt | learned:<file> | known:<case>]
 //   adapt            after the load steps (P1), adapt once with the nested indicators
 //                    (get_iso_target_size, adapt) and solve the last step again there
 //   learned:<file>   the learned indicator on the final state; <file> holds the
 //                    network as raw float32: GOAL_MLP_PARAMS parameters, then
 //                    GOAL_MLP_NORMS normalization values
+//   known:<case>     the settings of one of the reference's regression inputs
+//                    (unit cube, rollers on xmin/ymin/zmin, Newton tolerance
+//                    1e-8 in at most 5 iterations): nh_uniaxial
+//                    (example/primal/neohookean_uniaxial_3D.yaml), j2_uniaxial
+//                    (J2_uniaxial_3D.yaml) or j2_traction (J2_traction_3D.yaml,
+//                    a traction 1.0*t in y on ymax), in place of the default
+//                    zmin/zmax compression
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -80,12 +87,28 @@ int main(int argc, char** argv) {
     goal_material mat{};
     if (model == 2) mat = goal_material{GOAL_MODEL_J2, 200.0, 0.3, 2.0, 0.25, 1.0};
     else mat = goal_material{GOAL_MODEL_NEOHOOKEAN, 1000.0, 0.25, 0.0, 0.0, 1.0};
-    mp.materials = {mat};
-    goal::Mechanics mech(mp, disc.get());
     goal::PrimalParams pp;
     pp.dirichlet = {{0, "zmin", "0.0"}, {1, "zmin", "0.0"}, {2, "zmin", "0.0"}, {2, "zmax", "0.01*t"}};
     pp.max_iters = 10;
     pp.tolerance = 1e-10;
+    if (extra.rfind("known:", 0) == 0) {
+      const std::string kc = extra.substr(6);
+      if (kc != "nh_uniaxial" && kc != "j2_uniaxial" && kc != "j2_traction") {
+        std::fprintf(stderr, "unknown case %s\n", kc.c_str());
+        return 2;
+      }
+      if (kc != "nh_uniaxial") mat = goal_material{GOAL_MODEL_J2, 1000.0, 0.25, 100.0, 10.0, 1.0};
+      mp.model = kc == "nh_uniaxial" ? "neohookean" : "J2";
+      pp.dirichlet = {{0, "xmin", "0.0"}, {1, "ymin", "0.0"}, {2, "zmin", "0.0"}};
+      if (kc == "j2_traction")
+        pp.traction = {{"ymax", {"0.0", "1.0*t", "0.0"}}};
+      else
+        pp.dirichlet.push_back({0, "xmax", "0.01*t"});
+      pp.max_iters = 5;
+      pp.tolerance = 1e-8;
+    }
+    mp.materials = {mat};
+    goal::Mechanics mech(mp, disc.get());
     goal::GmresSolve gm;
     gm.tolerance = 1e-12;
     gm.max_iters = 5000;
@@ -107,7 +130,7 @@ int main(int argc, char** argv) {
     }
     std::printf("], \"J\": %.17g, \"norm\": %.6e", J, primal.last_norm());
     if (order == 1) {
-      goal::NestedAdjoint adj(&mech, "avg disp", -1, pp.dirichlet, solve);
+      goal::NestedAdjoint adj(&mech, "avg disp", -1, pp.dirichlet, solve, pp.traction);
       std::vector<double> eta = adj.run(t_old, t_old - 1.0);
       double s = 0.0;
       for (double e : eta) s += e;

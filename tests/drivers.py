@@ -75,7 +75,9 @@ def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0
     return out
 
 
-def nested_estimate(coords, conn, xb, st_base, model, mats, quad, t):
+def nested_estimate(coords, conn, xb, st_base, model, mats, quad, t, bcs=dbcs, tbc=None):
+    """tbc: None or (node set of the loaded face as a function of the
+    coordinates, T(x, y, z, t)) -- a traction on that face's boundary faces."""
     fc, fconn, parent, ep = g.mesh_refine(coords, conn)
     m = O.Mesh(1, fc, fconn)
     nq = 1 if quad == 1 else 4
@@ -91,9 +93,12 @@ def nested_estimate(coords, conn, xb, st_base, model, mats, quad, t):
     st.Fp[:] = st.Fp_old
     st.eqps_old[:] = st_base.eqps[src]
     st.eqps[:] = st.eqps_old
+    faces = O.boundary_faces(m, tbc[0](fc)) if tbc else None
     R, VT = O.assemble(O.K_ADJOINT, m, model, mats, x, st, quad_order=quad, save=False)
+    if tbc:
+        O.traction(m, faces, tbc[1], t, R)
     _, _, dM = O.qoi(0, m, model, mats, x, st, quad_order=quad, derivative=True)
-    for comp, nodes, vals in dbcs(fc, t):
+    for comp, nodes, vals in bcs(fc, t):
         O.dbc(m, nodes, vals, comp, x, R, dM, VT)
     z = spla.spsolve(csr(m, VT).tocsc(), dM)
     zR = -(R @ z)
@@ -101,12 +106,131 @@ def nested_estimate(coords, conn, xb, st_base, model, mats, quad, t):
     O.set_coarse(zc, ep, coords.shape[0])
     zd = z - zc
     e, _ = O.assemble(O.K_LOCALIZE, m, model, mats, x, st, quad_order=quad, save=False, z_diff=zd, z_coarse=zc)
-    for comp, nodes, vals in dbcs(fc, t):
+    if tbc:
+        O.traction(m, faces, tbc[1], t, e, z=zd)
+    for comp, nodes, vals in bcs(fc, t):
         O.dbc(m, nodes, vals, comp, x, e)
     bound = O.sum_contribs(np.arange(m.num_nodes), e)
     eta = O.compute_error(m, e)
     base = O.set_error(parent, eta, conn.shape[0])
     return {"eta_sum": float(base.sum()), "bound": float(bound), "zR": float(zR)}
+
+
+# The reference's 3D regression inputs on this path (unit cube, rollers on
+# xmin / ymin / zmin, 3 load steps of dt = 1, Newton tolerance 1e-8 in at most
+# 5 iterations) and their known final avg-disp values.
+KNOWN = {
+    "nh_uniaxial": dict(model=1, mat=[1000.0, 0.25, 0.0, 0.0, 1.0], traction=False, value=2.541285341193943e-03,
+                        src="example/primal/neohookean_uniaxial_3D.yaml:1-36"),
+    "j2_uniaxial": dict(model=2, mat=[1000.0, 0.25, 100.0, 10.0, 1.0], traction=False, value=1.073955612775838e-03,
+                        src="example/primal/J2_uniaxial_3D.yaml:1-38"),
+    "j2_traction": dict(model=2, mat=[1000.0, 0.25, 100.0, 10.0, 1.0], traction=True, value=2.512233163668167e-04,
+                        src="example/primal/J2_traction_3D.yaml:1-39"),
+}
+KNOWN_STEPS, KNOWN_TOL, KNOWN_MAX_ITERS = 3, 1e-8, 5
+
+
+def known_traction(x, y, z, t):
+    return (0.0, 1.0 * t, 0.0)
+
+
+def known_bcs(case):
+    def bcs(coords, t):
+        xmin, ymin, zmin = face_nodes(coords, 0, 0.0), face_nodes(coords, 1, 0.0), face_nodes(coords, 2, 0.0)
+        out = [(0, xmin, np.zeros(xmin.size)), (1, ymin, np.zeros(ymin.size)), (2, zmin, np.zeros(zmin.size))]
+        if not KNOWN[case]["traction"]:
+            xmax = face_nodes(coords, 0, 1.0)
+            out.append((0, xmax, np.full(xmax.size, 0.01 * t)))
+        return out
+    return bcs
+
+
+def known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, dwr=False):
+    """Oracle Newton on one of the reference's regression inputs; the mesh
+    is a jittered n^3 box (the solutions are homogeneous, so any mesh of the
+    unit cube reproduces the reference's value)."""
+    k = KNOWN[case]
+    coords, conn = g.mesh_box(n, n, n, order=order, jitter=jitter, seed=seed)
+    m = O.Mesh(order, coords, conn)
+    model, mats = k["model"], [k["mat"]]
+    bcs = known_bcs(case)
+    faces = O.boundary_faces(m, face_nodes(coords, 1, 1.0)) if k["traction"] else None
+    st = O.States(m, model, quad)
+    x = np.zeros(m.num_dofs)
+    iters, J, norm = [], 0.0, 0.0
+    for step in range(1, KNOWN_STEPS + 1):
+        t = float(step)
+        for it in range(1, KNOWN_MAX_ITERS + 1):
+            R, V = O.assemble(O.K_JACOBIAN, m, model, mats, x, st, quad_order=quad)
+            if faces is not None:
+                O.traction(m, faces, known_traction, t, R)
+            for comp, nodes, vals in bcs(coords, t):
+                O.dbc(m, nodes, vals, comp, x, R, np.zeros(m.num_dofs), V)
+            x = x + spla.spsolve(csr(m, V).tocsc(), -R)
+            R, _ = O.assemble(O.K_RESIDUAL, m, model, mats, x, st, quad_order=quad)
+            if faces is not None:
+                O.traction(m, faces, known_traction, t, R)
+            for comp, nodes, vals in bcs(coords, t):
+                O.dbc(m, nodes, vals, comp, x, R)
+            norm = np.linalg.norm(R)
+            if norm < KNOWN_TOL:
+                break
+        else:
+            raise RuntimeError("newton failed")
+        iters.append(it)
+        J, _, _ = O.qoi(0, m, model, mats, x, st, quad_order=quad)
+        if step < KNOWN_STEPS:
+            st.update()
+    out = {"iters": iters, "J": J, "norm": norm}
+    if dwr and order == 1:
+        tbc = (lambda c: face_nodes(c, 1, 1.0), known_traction) if k["traction"] else None
+        out.update(nested_estimate(coords, conn, x, st, model, mats, quad, float(KNOWN_STEPS), bcs=bcs, tbc=tbc))
+    return out
+
+
+def gpu_known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, lin_rtol=1e-13):
+    """The same on the GPU through the C ABI: device assembly, Dirichlet rows,
+    traction loads and norms, the library's device GMRES."""
+    k = KNOWN[case]
+    disc = g.Disc.box(n, n, n, order=order, jitter=jitter, seed=seed)
+    model = k["model"]
+    st = g.States(disc, model, quad)
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=model, **dict(zip(("E", "nu", "K", "Y", "c0"), k["mat"])))],
+                       stabilization=True, quad_order=quad)
+    si = g.SolInfo(disc)
+    bcs = [(0, "xmin", "0.0"), (1, "ymin", "0.0"), (2, "zmin", "0.0")]
+    if not k["traction"]:
+        bcs.append((0, "xmax", "0.01*t"))
+    b, du = g.DeviceArray(disc.num_dofs), g.DeviceArray(disc.num_dofs)
+    iters, norm = [], 0.0
+    for step in range(1, KNOWN_STEPS + 1):
+        t = float(step)
+        for it in range(1, KNOWN_MAX_ITERS + 1):
+            si.zero_all()
+            mech.jacobian(si, adjoint=False, save_states=True)
+            if k["traction"]:
+                g.tbc_apply(disc, si, "ymax", ("0.0", "1.0*t", "0.0"), t)
+            for comp, ns, expr in bcs:
+                g.dbc_apply(disc, si, comp, ns, expr, t, True)
+            g.check(g.lib.goal_memcpy_h2d(b.ptr, g._ptr(np.ascontiguousarray(-si.R())), 8 * disc.num_dofs))
+            du.fill(0.0)
+            si.solve(b, du, rtol=lin_rtol, max_iters=5000, restart=100)
+            disc.add_soln_dev(du)
+            si.zero_R()
+            mech.residual(si, save_states=True)
+            if k["traction"]:
+                g.tbc_apply(disc, si, "ymax", ("0.0", "1.0*t", "0.0"), t)
+            for comp, ns, expr in bcs:
+                g.dbc_apply(disc, si, comp, ns, expr, t, False)
+            norm = si.norm_R()
+            if norm < KNOWN_TOL:
+                break
+        else:
+            raise RuntimeError("newton failed")
+        iters.append(it)
+        if step < KNOWN_STEPS:
+            st.update()
+    return {"iters": iters, "J": mech.qoi(0), "norm": norm}
 
 
 def gpu_newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0.1, seed=11, tangent=64,
