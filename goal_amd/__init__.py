@@ -145,6 +145,14 @@ _sig("goal_dbc_apply", _i32, _p, _p, _i32, C.c_char_p, C.c_char_p, _d, _i32)
 _sig("goal_dbc_apply_values", _i32, _p, _p, _i32, _i64, _p, _p, _i32)
 _sig("goal_tbc_apply", _i32, _p, _p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, _d, _p)
 _sig("goal_disc_add_side_set", _i32, _p, C.c_char_p, _i64, _p, _p)
+_sig("goal_mesh_file_read", _i32, C.c_char_p, C.c_char_p, C.c_char_p, _i32, C.POINTER(_p))
+_sig("goal_mesh_file_destroy", _i32, _p)
+_sig("goal_mesh_file_sizes", _i32, _p, _i64p, _i64p, C.POINTER(C.c_int))
+_sig("goal_mesh_file_get_mesh", _i32, _p, _p, _p, _p)
+_sig("goal_mesh_file_elem_set_name", _i32, _p, _i32, C.c_char_p, _i64)
+_sig("goal_mesh_file_node_set", _i32, _p, C.c_char_p, _i64p, _p)
+_sig("goal_mesh_file_side_set", _i32, _p, C.c_char_p, _i64p, _p, _p)
+_sig("goal_disc_create_from_file", _i32, _p, C.POINTER(_p))
 _sig("goal_disc_side_set_size", _i32, _p, C.c_char_p, _i64p)
 _sig("goal_disc_get_side_set", _i32, _p, C.c_char_p, _p, _p)
 _sig("goal_batch_create", _i32, _i32, _p, _p, C.POINTER(_p))
@@ -741,6 +749,57 @@ def dbc_apply_values(disc: Disc, s: SolInfo, comp: int, nodes, values, jacobian:
     nodes = np.ascontiguousarray(nodes, dtype=np.int32)
     values = np.ascontiguousarray(values, dtype=np.float64)
     check(lib.goal_dbc_apply_values(disc.h, s.h, comp, nodes.size, _ptr(nodes), _ptr(values), int(bool(jacobian))))
+
+
+class MeshFile:
+    """goal's mesh inputs (serial SCOREC .smb, gmi .dmg, association file)
+    read on the host (src/goal_disc.cpp:28-99, 347-396); needs no GPU."""
+
+    def __init__(self, geom_file, mesh_file, assoc_file, order=1):
+        self.h = _p()
+        check(lib.goal_mesh_file_read(str(geom_file).encode(), str(mesh_file).encode(), str(assoc_file).encode(),
+                                      order, C.byref(self.h)))
+        nn, ne, ns = C.c_int64(0), C.c_int64(0), C.c_int(0)
+        check(lib.goal_mesh_file_sizes(self.h, C.byref(nn), C.byref(ne), C.byref(ns)))
+        self.num_nodes, self.num_elems, self.num_elem_sets = nn.value, ne.value, ns.value
+        self.order = order
+        self.elem_set_names = []
+        for i in range(self.num_elem_sets):
+            buf = C.create_string_buffer(256)
+            check(lib.goal_mesh_file_elem_set_name(self.h, i, buf, 256))
+            self.elem_set_names.append(buf.value.decode())
+
+    def mesh(self):
+        coords = np.empty((self.num_nodes, 3))
+        conn = np.empty((self.num_elems, 4 if self.order == 1 else 10), dtype=np.int32)
+        es = np.empty(self.num_elems, dtype=np.int32)
+        check(lib.goal_mesh_file_get_mesh(self.h, _ptr(coords), _ptr(conn), _ptr(es)))
+        return coords, conn, es
+
+    def node_set(self, name: str) -> np.ndarray:
+        n = C.c_int64(0)
+        check(lib.goal_mesh_file_node_set(self.h, name.encode(), C.byref(n), None))
+        out = np.empty(n.value, dtype=np.int32)
+        check(lib.goal_mesh_file_node_set(self.h, name.encode(), C.byref(n), _ptr(out)))
+        return out
+
+    def side_set(self, name: str):
+        n = C.c_int64(0)
+        check(lib.goal_mesh_file_side_set(self.h, name.encode(), C.byref(n), None, None))
+        elems = np.empty(n.value, dtype=np.int32)
+        faces = np.empty(n.value, dtype=np.int32)
+        check(lib.goal_mesh_file_side_set(self.h, name.encode(), C.byref(n), _ptr(elems), _ptr(faces)))
+        return elems, faces
+
+    def disc(self) -> "Disc":
+        h = _p()
+        check(lib.goal_disc_create_from_file(self.h, C.byref(h)))
+        return Disc(h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.goal_mesh_file_destroy(self.h)
+            self.h = _p()
 
 
 def tbc_apply(disc: Disc, s: SolInfo, side_set: str, exprs, t: float, weight=None) -> None:

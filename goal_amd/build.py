@@ -45,6 +45,7 @@ HIP_FLAGS = [
 SOURCES = [
     "goal_mesh.cpp",
     "goal_adapt.cpp",
+    "goal_meshio.cpp",
     "goal_expr.cpp",
     "goal_capi.cpp",
     "goal_assembly.hip",

@@ -23,6 +23,7 @@
 #include "goal_expr.hpp"
 #include "goal_kernels.hpp"
 #include "goal_mesh.hpp"
+#include "goal_meshio.hpp"
 
 namespace {
 
@@ -812,6 +813,96 @@ int goal_disc_create_box(int nx, int ny, int nz, double lx, double ly, double lz
     require(out != nullptr, "goal_disc_create_box: null output");
     ctx();
     *out = new_disc(goal::make_box_mesh(nx, ny, nz, lx, ly, lz, order, jitter, seed));
+    return GOAL_OK;
+  });
+}
+
+// ------------------------------------------------------------ mesh files
+struct goal_mesh_file_s {
+  goal::MeshFile f;
+};
+
+int goal_mesh_file_read(const char* geom_file, const char* mesh_file, const char* assoc_file, int order,
+                        goal_mesh_file* out) {
+  return guard([&]() -> int {
+    require(geom_file && mesh_file && assoc_file && out, "goal_mesh_file_read: bad arguments");
+    std::unique_ptr<goal_mesh_file_s> f(new goal_mesh_file_s);
+    f->f = goal::read_mesh_files(geom_file, mesh_file, assoc_file, order);
+    *out = f.release();
+    return GOAL_OK;
+  });
+}
+
+int goal_mesh_file_destroy(goal_mesh_file f) {
+  delete f;
+  return GOAL_OK;
+}
+
+int goal_mesh_file_sizes(goal_mesh_file f, int64_t* num_nodes, int64_t* num_elems, int* num_elem_sets) {
+  return guard([&]() -> int {
+    require(f != nullptr, "null mesh file");
+    if (num_nodes) *num_nodes = f->f.mesh.num_nodes;
+    if (num_elems) *num_elems = f->f.mesh.num_elems;
+    if (num_elem_sets) *num_elem_sets = (int)f->f.elem_set_names.size();
+    return GOAL_OK;
+  });
+}
+
+int goal_mesh_file_get_mesh(goal_mesh_file f, double* coords, int32_t* conn, int32_t* elem_set) {
+  return guard([&]() -> int {
+    require(f != nullptr, "null mesh file");
+    const goal::HostMesh& m = f->f.mesh;
+    if (coords) std::copy(m.coords.begin(), m.coords.end(), coords);
+    if (conn) std::copy(m.conn.begin(), m.conn.end(), conn);
+    if (elem_set) std::copy(m.elem_set.begin(), m.elem_set.end(), elem_set);
+    return GOAL_OK;
+  });
+}
+
+int goal_mesh_file_elem_set_name(goal_mesh_file f, int index, char* buf, int64_t len) {
+  return guard([&]() -> int {
+    require(f && buf && len > 0, "goal_mesh_file_elem_set_name: bad arguments");
+    require(index >= 0 && index < (int)f->f.elem_set_names.size(), "element set index out of range");
+    const std::string& n = f->f.elem_set_names[index];
+    require((int64_t)n.size() < len, "name buffer too small");
+    std::memcpy(buf, n.c_str(), n.size() + 1);
+    return GOAL_OK;
+  });
+}
+
+int goal_mesh_file_node_set(goal_mesh_file f, const char* name, int64_t* n, int32_t* nodes) {
+  return guard([&]() -> int {
+    require(f && name && n, "goal_mesh_file_node_set: bad arguments");
+    auto it = f->f.mesh.node_sets.find(name);
+    if (it == f->f.mesh.node_sets.end()) return set_error(GOAL_ERR_INVALID, "unknown node set: %s", name);
+    *n = (int64_t)it->second.size();
+    if (nodes) std::copy(it->second.begin(), it->second.end(), nodes);
+    return GOAL_OK;
+  });
+}
+
+int goal_mesh_file_side_set(goal_mesh_file f, const char* name, int64_t* n, int32_t* elems, int32_t* faces) {
+  return guard([&]() -> int {
+    require(f && name && n, "goal_mesh_file_side_set: bad arguments");
+    auto it = f->f.side_sets.find(name);
+    if (it == f->f.side_sets.end()) return set_error(GOAL_ERR_INVALID, "unknown side set: %s", name);
+    *n = (int64_t)it->second.size();
+    for (size_t k = 0; k < it->second.size(); ++k) {
+      if (elems) elems[k] = it->second[k] / 4;
+      if (faces) faces[k] = it->second[k] % 4;
+    }
+    return GOAL_OK;
+  });
+}
+
+int goal_disc_create_from_file(goal_mesh_file f, goal_disc* out) {
+  return guard([&]() -> int {
+    require(f && out, "goal_disc_create_from_file: bad arguments");
+    ctx();
+    goal::HostMesh m = f->f.mesh;
+    goal_disc d = new_disc(std::move(m));
+    d->side_sets = f->f.side_sets;
+    *out = d;
     return GOAL_OK;
   });
 }

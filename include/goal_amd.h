@@ -111,6 +111,29 @@ int goal_disc_create(int order, int64_t num_nodes, const double* coords, int64_t
  * jittered by +-jitter*h (seeded), node sets xmin..zmax on the closed faces. */
 int goal_disc_create_box(int nx, int ny, int nz, double lx, double ly, double lz, int order, double jitter,
                          uint64_t seed, goal_disc* out);
+/* goal's mesh inputs, read on the host: apf::loadMdsMesh(geom file, mesh
+ * file) + read_sets(assoc file) (src/goal_disc.cpp:28-99), then the sets of
+ * compute_elem_sets / compute_side_sets / compute_node_sets
+ * (src/goal_disc.cpp:347-396).  mesh_file: a one-part SCOREC MDS file of
+ * version 5 holding tetrahedra (the reference's *-serial0.smb meshes);
+ * geom_file: its gmi .dmg model; assoc_file: "node set / side set / elem set"
+ * lines with (model dim, model tag) entries.  Node sets hold the nodes
+ * classified in the closure of their model entities; side sets the triangles
+ * classified on their model faces, as (element, local face); element sets
+ * are numbered in file order.  order 2 adds a node at each edge midpoint. */
+typedef struct goal_mesh_file_s* goal_mesh_file;
+int goal_mesh_file_read(const char* geom_file, const char* mesh_file, const char* assoc_file, int order,
+                        goal_mesh_file* out);
+int goal_mesh_file_destroy(goal_mesh_file f);
+int goal_mesh_file_sizes(goal_mesh_file f, int64_t* num_nodes, int64_t* num_elems, int* num_elem_sets);
+/* coords [num_nodes][3], conn [num_elems][4 or 10], elem_set [num_elems]; any may be NULL */
+int goal_mesh_file_get_mesh(goal_mesh_file f, double* coords, int32_t* conn, int32_t* elem_set);
+int goal_mesh_file_elem_set_name(goal_mesh_file f, int index, char* buf, int64_t len);
+/* *n = the set's size; nodes / elems, faces (may be NULL) receive it */
+int goal_mesh_file_node_set(goal_mesh_file f, const char* name, int64_t* n, int32_t* nodes);
+int goal_mesh_file_side_set(goal_mesh_file f, const char* name, int64_t* n, int32_t* elems, int32_t* faces);
+/* The Disc of a read mesh file, with its node, side and element sets. */
+int goal_disc_create_from_file(goal_mesh_file f, goal_disc* out);
 /* destroy_disc (src/goal_disc.cpp:452-454) */
 int goal_disc_destroy(goal_disc d);
 /* out[0..8] = num_nodes, num_elems, num_dofs, nnz, nodes_per_elem, num_vertices, order,

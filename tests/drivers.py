@@ -134,27 +134,39 @@ def known_traction(x, y, z, t):
     return (0.0, 1.0 * t, 0.0)
 
 
-def known_bcs(case):
+def known_bcs(case, sets=None):
+    """The regression inputs' Dirichlet entries; sets: node sets by name (the
+    mesh file's), else the box faces found from the coordinates."""
     def bcs(coords, t):
-        xmin, ymin, zmin = face_nodes(coords, 0, 0.0), face_nodes(coords, 1, 0.0), face_nodes(coords, 2, 0.0)
+        if sets is None:
+            xmin, ymin, zmin = face_nodes(coords, 0, 0.0), face_nodes(coords, 1, 0.0), face_nodes(coords, 2, 0.0)
+            xmax = face_nodes(coords, 0, 1.0)
+        else:
+            xmin, ymin, zmin, xmax = sets["xmin"], sets["ymin"], sets["zmin"], sets["xmax"]
         out = [(0, xmin, np.zeros(xmin.size)), (1, ymin, np.zeros(ymin.size)), (2, zmin, np.zeros(zmin.size))]
         if not KNOWN[case]["traction"]:
-            xmax = face_nodes(coords, 0, 1.0)
             out.append((0, xmax, np.full(xmax.size, 0.01 * t)))
         return out
     return bcs
 
 
-def known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, dwr=False):
-    """Oracle Newton on one of the reference's regression inputs; the mesh
-    is a jittered n^3 box (the solutions are homogeneous, so any mesh of the
-    unit cube reproduces the reference's value)."""
+def known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, dwr=False, mesh=None):
+    """Oracle Newton on one of the reference's regression inputs.  The mesh is
+    a jittered n^3 box (the solutions are homogeneous, so any mesh of the unit
+    cube reproduces the reference's value), or mesh = (coords, conn, node sets
+    by name, ymax faces as node lists) from a mesh file."""
     k = KNOWN[case]
-    coords, conn = g.mesh_box(n, n, n, order=order, jitter=jitter, seed=seed)
+    if mesh is None:
+        coords, conn = g.mesh_box(n, n, n, order=order, jitter=jitter, seed=seed)
+        sets = None
+    else:
+        coords, conn, sets, ymax_faces = mesh
     m = O.Mesh(order, coords, conn)
     model, mats = k["model"], [k["mat"]]
-    bcs = known_bcs(case)
-    faces = O.boundary_faces(m, face_nodes(coords, 1, 1.0)) if k["traction"] else None
+    bcs = known_bcs(case, sets)
+    faces = None
+    if k["traction"]:
+        faces = O.boundary_faces(m, face_nodes(coords, 1, 1.0)) if mesh is None else ymax_faces
     st = O.States(m, model, quad)
     x = np.zeros(m.num_dofs)
     iters, J, norm = [], 0.0, 0.0
@@ -182,17 +194,20 @@ def known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, dwr=False):
         if step < KNOWN_STEPS:
             st.update()
     out = {"iters": iters, "J": J, "norm": norm}
-    if dwr and order == 1:
+    if dwr and order == 1 and mesh is None:
         tbc = (lambda c: face_nodes(c, 1, 1.0), known_traction) if k["traction"] else None
         out.update(nested_estimate(coords, conn, x, st, model, mats, quad, float(KNOWN_STEPS), bcs=bcs, tbc=tbc))
     return out
 
 
-def gpu_known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, lin_rtol=1e-13):
+def gpu_known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, lin_rtol=1e-13, disc=None):
     """The same on the GPU through the C ABI: device assembly, Dirichlet rows,
-    traction loads and norms, the library's device GMRES."""
+    traction loads and norms, the library's device GMRES.  disc: a Disc with
+    node sets xmin, ymin, zmin, xmax and side set ymax (e.g. read from the
+    reference's mesh files), else a jittered n^3 box."""
     k = KNOWN[case]
-    disc = g.Disc.box(n, n, n, order=order, jitter=jitter, seed=seed)
+    if disc is None:
+        disc = g.Disc.box(n, n, n, order=order, jitter=jitter, seed=seed)
     model = k["model"]
     st = g.States(disc, model, quad)
     mech = g.Mechanics(disc, st, [g.MaterialSpec(model=model, **dict(zip(("E", "nu", "K", "Y", "c0"), k["mat"])))],

@@ -109,3 +109,87 @@ def test_boundary_faces_of_box_face():
                    for e in range(m.num_elems) for fv in O._TET_FACES)
     assert on_plane == 2 * (2 * 3 * 2)   # 12 faces on the plane, each seen from both sides
     assert O.boundary_faces(m, mid).shape[0] == 0
+
+
+# --------------------------------------------------- the reference's own mesh
+GOLDEN_SMB = __import__("pathlib").Path(__file__).resolve().parent / "golden" / "smb"
+
+
+def _cube_file(order=1):
+    import goal_amd as g
+    return g.MeshFile(GOLDEN_SMB / "cube.dmg", GOLDEN_SMB / "cube-serial0.smb", GOLDEN_SMB / "cube.txt", order=order)
+
+
+def faces_as_nodes(conn, order, elems, faces):
+    """(element, local face) pairs as face node lists (3 vertices, then for
+    P2 the nodes of edges v0v1, v1v2, v2v0), in apf's tet face order."""
+    import oracle as O
+    out = []
+    for e, f in zip(elems, faces):
+        lv = O._TET_FACES[f]
+        row = [int(conn[e, v]) for v in lv]
+        if order == 2:
+            for j in range(3):
+                pair = {lv[j], lv[(j + 1) % 3]}
+                row.append(int(conn[e, 4 + next(i for i, ed in enumerate(O._TET_EDGES) if set(ed) == pair)]))
+        out.append(row)
+    return np.asarray(out, dtype=np.int32)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_mesh_file_reader_on_reference_cube(order):
+    """The reference's test mesh (test/mesh/cube: cube-serial0.smb, cube.dmg,
+    cube.txt, committed as fixtures) read by the library on the host: 51
+    vertices, 230 edges, 132 positively oriented tets filling the unit cube;
+    the node sets (closure of their model faces) are exactly the nodes on
+    those faces; the side sets are the face triangles, area 1 each; one
+    element set 'cube'."""
+    f = _cube_file(order)
+    coords, conn, es = f.mesh()
+    assert (f.num_nodes, f.num_elems) == ((51, 132) if order == 1 else (51 + 230, 132))
+    assert f.elem_set_names == ["cube"] and np.all(es == 0)
+    X = coords[conn[:, :4]]
+    vol = np.einsum("ij,ij->i", X[:, 1] - X[:, 0], np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0])) / 6.0
+    assert vol.min() > 0 and abs(vol.sum() - 1.0) < 1e-14
+    if order == 2:   # edge nodes at the midpoints of their tet edges
+        import oracle as O
+        for ed, (a, b) in enumerate(O._TET_EDGES):
+            assert np.allclose(coords[conn[:, 4 + ed]], 0.5 * (coords[conn[:, a]] + coords[conn[:, b]]), atol=1e-15)
+    for name, comp, val in (("xmin", 0, 0.0), ("xmax", 0, 1.0), ("ymin", 1, 0.0), ("zmin", 2, 0.0),
+                            ("zmax", 2, 1.0)):
+        assert np.array_equal(np.sort(f.node_set(name)), np.nonzero(coords[:, comp] == val)[0]), name
+    for name, comp, val in (("ymax", 1, 1.0), ("xmax", 0, 1.0)):
+        elems, faces = f.side_set(name)
+        fn = faces_as_nodes(conn, order, elems, faces)
+        assert np.all(coords[fn.ravel(), comp] == val)
+        a = coords[fn[:, 1]] - coords[fn[:, 0]]
+        b = coords[fn[:, 2]] - coords[fn[:, 0]]
+        assert abs(0.5 * np.linalg.norm(np.cross(a, b), axis=1).sum() - 1.0) < 1e-14, name
+
+
+def test_mesh_file_reader_rejects_bad_input(tmp_path):
+    import goal_amd as g
+    raw = (GOLDEN_SMB / "cube-serial0.smb").read_bytes()
+    bad_version = raw[:4] + (6).to_bytes(4, "big") + raw[8:]
+    (tmp_path / "v6.smb").write_bytes(bad_version)
+    (tmp_path / "short.smb").write_bytes(raw[:5000])
+    for name in ("v6.smb", "short.smb"):
+        with pytest.raises(g.GoalError):
+            g.MeshFile(GOLDEN_SMB / "cube.dmg", tmp_path / name, GOLDEN_SMB / "cube.txt")
+    (tmp_path / "bad.txt").write_text("node set xmin 1\n2 99\n")
+    with pytest.raises(g.GoalError, match="no model entity"):
+        g.MeshFile(GOLDEN_SMB / "cube.dmg", GOLDEN_SMB / "cube-serial0.smb", tmp_path / "bad.txt")
+
+
+@pytest.mark.parametrize("case", sorted(drivers.KNOWN))
+def test_oracle_reproduces_known_answer_on_reference_mesh(case):
+    """The regression inputs as the reference runs them: its cube mesh, model
+    and association file (read by the library), its node sets and side set,
+    P1 with one point.  The oracle's Newton gives the stored values."""
+    f = _cube_file(1)
+    coords, conn, _ = f.mesh()
+    sets = {k: f.node_set(k) for k in ("xmin", "ymin", "zmin", "xmax")}
+    ymax = faces_as_nodes(conn, 1, *f.side_set("ymax"))
+    out = drivers.known_newton(case, mesh=(coords, conn, sets, ymax))
+    print(case, out["iters"], out["J"] - drivers.KNOWN[case]["value"])
+    assert abs(out["J"] - drivers.KNOWN[case]["value"]) < ABS_TOL
