@@ -33,6 +33,8 @@ struct Quad<1> {
     xi[0] = xi[1] = xi[2] = 0.25;
     w = 1.0 / 6.0;
   }
+  // xi[c] of point q, as point() sets it
+  __device__ static inline double coord(int, int) { return 0.25; }
 };
 template <>
 struct Quad<4> {
@@ -42,6 +44,10 @@ struct Quad<4> {
     xi[1] = q == 2 ? a : b;
     xi[2] = q == 3 ? a : b;
     w = 1.0 / 24.0;
+  }
+  __device__ static inline double coord(int q, int c) {
+    const double a = 0.5854101966249685, b = 0.1381966011250105;
+    return q == c + 1 ? a : b;
   }
 };
 
