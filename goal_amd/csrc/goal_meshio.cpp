@@ -4,15 +4,20 @@
 // compute_elem_sets / compute_side_sets / compute_node_sets turn them into
 // sets (src/goal_disc.cpp:347-396).
 //
-// The .smb layout read here (version 5, one part; all integers are 32-bit
-// unsigned and all reals 64-bit, both big-endian):
+// The .smb layout read here (version 5; all integers are 32-bit unsigned and
+// all reals 64-bit, both big-endian):
 //   magic (0), version, dimension, part count
 //   entity counts by type: vertex, edge, triangle, quad, hex, prism, pyramid, tet
 //   downward adjacency: edges -> 2 vertices, triangles -> 3 edges, tets -> 4 triangles
 //   vertex coordinates [num_vertices][3], vertex parametric coordinates [num_vertices][2]
-//   remote copies: peer count (0 on one part)
+//   remote copies of vertices: peer count P, the P peer parts, the P copy
+//   counts, then per peer the local ids of the shared vertices (the k-th id
+//   a part lists for peer p and the k-th id p lists for it are one vertex)
 //   classification: (model tag, model dimension) per entity, by type
 //   (tags and the coordinate field's shape follow and are not needed)
+// A partitioned mesh "name.smb" is stored as name0.smb ... name<P-1>.smb (the
+// part number goes before the extension, as apf writes and reads them); its
+// parts are merged into one mesh through the vertex copies.
 // The .dmg (gmi "mesh model") text: counts of regions, faces, edges and
 // vertices; two bounding-box lines; vertices (tag x y z); edges (tag, its two
 // vertex tags); faces (tag, loop count, then per loop its edge count and
@@ -25,6 +30,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <set>
 #include <sstream>
 #include <stdexcept>
@@ -66,20 +72,26 @@ struct Smb {
   std::vector<int32_t> edge_verts, tri_edges, tet_tris;  // downward adjacency
   std::vector<double> coords;
   std::vector<std::array<int32_t, 2>> cls[4];  // (dim, tag) of vertices, edges, triangles, tets
+  uint32_t parts = 1;
+  std::vector<std::pair<uint32_t, std::vector<int32_t>>> links;  // peer part, shared local vertex ids
 };
 
-Smb read_smb(const std::string& path) {
+bool exists(const std::string& path) { return std::ifstream(path).good(); }
+
+Smb read_smb(const std::string& path, uint32_t part = 0) {
   const std::vector<uint8_t> bytes = slurp(path);
   BigEndian in{bytes, 0, path};
   if (in.u32() != 0) fail(path + ": not an MDS mesh file (bad magic number)");
   const uint32_t version = in.u32();
   if (version != 5) fail(path + ": MDS file version " + std::to_string(version) + " (only version 5 is read)");
   if (in.u32() != 3) fail(path + ": not a 3D mesh");
-  if (in.u32() != 1) fail(path + ": a part of a partitioned mesh (only one-part meshes are read)");
+  const uint32_t parts = in.u32();
+  if (parts < 1 || part >= parts) fail(path + ": bad part count");
   uint32_t n[8];
   for (auto& c : n) c = in.u32();
   if (n[3] || n[4] || n[5] || n[6]) fail(path + ": has non-tetrahedral elements");
   Smb s;
+  s.parts = parts;
   s.nv = n[0];
   s.ne = n[1];
   s.nt = n[2];
@@ -99,7 +111,26 @@ Smb read_smb(const std::string& path) {
   s.coords.resize(3 * s.nv);
   for (auto& x : s.coords) x = in.f64();
   for (int64_t k = 0; k < 2 * s.nv; ++k) in.f64();  // parametric coordinates
-  if (in.u32() != 0) fail(path + ": has remote copies (only one-part meshes are read)");
+  const uint32_t npeers = in.u32();
+  if (npeers >= parts) fail(path + ": more peers than parts");
+  s.links.resize(npeers);
+  for (auto& l : s.links) {
+    l.first = in.u32();
+    if (l.first >= parts || l.first == part) fail(path + ": bad peer part");
+  }
+  std::vector<uint32_t> counts(npeers);
+  for (auto& c : counts) {
+    c = in.u32();
+    if (c > s.nv) fail(path + ": more copies than vertices");
+  }
+  for (uint32_t k = 0; k < npeers; ++k) {
+    s.links[k].second.resize(counts[k]);
+    for (auto& v : s.links[k].second) {
+      const uint32_t x = in.u32();
+      if (x >= s.nv) fail(path + ": remote copy of a vertex out of range");
+      v = (int32_t)x;
+    }
+  }
   const int64_t cnt[4] = {s.nv, s.ne, s.nt, s.ntet};
   for (int d = 0; d < 4; ++d) {
     s.cls[d].resize(cnt[d]);
@@ -110,6 +141,106 @@ Smb read_smb(const std::string& path) {
     }
   }
   return s;
+}
+
+// The parts of a partitioned mesh as one mesh: vertices joined through their
+// remote copies (first copy's coordinates and classification), edges and
+// triangles joined by their vertices, tets in part order.
+Smb merge_parts(const std::vector<Smb>& P, const std::string& name) {
+  const size_t np = P.size();
+  std::vector<int64_t> base(np + 1, 0);
+  for (size_t a = 0; a < np; ++a) base[a + 1] = base[a] + P[a].nv;
+  std::vector<int64_t> up(base[np]);
+  for (int64_t k = 0; k < base[np]; ++k) up[k] = k;
+  std::function<int64_t(int64_t)> root = [&](int64_t k) {
+    while (up[k] != k) k = up[k] = up[up[k]];
+    return k;
+  };
+  for (size_t a = 0; a < np; ++a)
+    for (auto const& l : P[a].links) {
+      const Smb& q = P[l.first];
+      const std::vector<int32_t>* back = nullptr;
+      for (auto const& m : q.links)
+        if (m.first == a) back = &m.second;
+      if (!back || back->size() != l.second.size()) fail(name + ": remote copies of parts do not pair up");
+      for (size_t k = 0; k < l.second.size(); ++k) {
+        const int64_t x = root(base[a] + l.second[k]), y = root(base[l.first] + (*back)[k]);
+        if (x != y) up[std::max(x, y)] = std::min(x, y);
+      }
+    }
+  Smb m;
+  std::vector<int32_t> gid(base[np], -1);
+  for (size_t a = 0; a < np; ++a)
+    for (int64_t v = 0; v < P[a].nv; ++v) {
+      const int64_t r = root(base[a] + v);
+      if (gid[r] < 0) {
+        gid[r] = (int32_t)m.nv++;
+        for (int c = 0; c < 3; ++c) m.coords.push_back(P[a].coords[3 * v + c]);
+        m.cls[0].push_back(P[a].cls[0][v]);
+      }
+      gid[base[a] + v] = gid[r];
+    }
+  std::map<std::pair<int32_t, int32_t>, int32_t> edges;
+  std::map<std::array<int32_t, 3>, int32_t> tris;
+  for (size_t a = 0; a < np; ++a) {
+    const Smb& q = P[a];
+    std::vector<int32_t> eid(q.ne), tid(q.nt);
+    for (int64_t e = 0; e < q.ne; ++e) {
+      const int32_t x = gid[base[a] + q.edge_verts[2 * e]], y = gid[base[a] + q.edge_verts[2 * e + 1]];
+      auto it = edges.emplace(std::make_pair(std::min(x, y), std::max(x, y)), (int32_t)m.ne);
+      if (it.second) {
+        m.edge_verts.push_back(x);
+        m.edge_verts.push_back(y);
+        m.cls[1].push_back(q.cls[1][e]);
+        ++m.ne;
+      }
+      eid[e] = it.first->second;
+    }
+    for (int64_t t = 0; t < q.nt; ++t) {
+      std::array<int32_t, 3> ev = {eid[q.tri_edges[3 * t]], eid[q.tri_edges[3 * t + 1]], eid[q.tri_edges[3 * t + 2]]};
+      std::set<int32_t> vs;
+      for (int32_t e : ev) {
+        vs.insert(m.edge_verts[2 * e]);
+        vs.insert(m.edge_verts[2 * e + 1]);
+      }
+      if (vs.size() != 3) fail(name + ": a triangle's edges do not close");
+      std::array<int32_t, 3> key;
+      std::copy(vs.begin(), vs.end(), key.begin());
+      auto it = tris.emplace(key, (int32_t)m.nt);
+      if (it.second) {
+        for (int32_t e : ev) m.tri_edges.push_back(e);
+        m.cls[2].push_back(q.cls[2][t]);
+        ++m.nt;
+      }
+      tid[t] = it.first->second;
+    }
+    for (int64_t k = 0; k < q.ntet; ++k) {
+      for (int f = 0; f < 4; ++f) m.tet_tris.push_back(tid[q.tet_tris[4 * k + f]]);
+      m.cls[3].push_back(q.cls[3][k]);
+      ++m.ntet;
+    }
+  }
+  return m;
+}
+
+// One-part file as named, else the parts name0.smb ... of a partitioned mesh.
+Smb read_mesh(const std::string& path) {
+  if (exists(path)) {
+    Smb s = read_smb(path);
+    if (s.parts != 1) fail(path + ": one part of a partitioned mesh; name the mesh without its part number");
+    return s;
+  }
+  const size_t dot = path.rfind(".smb");
+  if (dot == std::string::npos || dot + 4 != path.size()) fail("cannot open " + path);
+  auto part_file = [&](uint32_t k) { return path.substr(0, dot) + std::to_string(k) + ".smb"; };
+  if (!exists(part_file(0))) fail("cannot open " + path + " or its part " + part_file(0));
+  std::vector<Smb> P;
+  P.push_back(read_smb(part_file(0), 0));
+  for (uint32_t k = 1; k < P[0].parts; ++k) {
+    P.push_back(read_smb(part_file(k), k));
+    if (P.back().parts != P[0].parts) fail(part_file(k) + ": part counts differ");
+  }
+  return P.size() == 1 ? P[0] : merge_parts(P, path);
 }
 
 // Model topology: the entities in the closure of each model entity.
@@ -221,7 +352,7 @@ constexpr int kFace[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
 MeshFile read_mesh_files(const std::string& geom_file, const std::string& mesh_file, const std::string& assoc_file,
                          int order) {
   if (order != 1 && order != 2) fail("mesh order must be 1 or 2");
-  const Smb s = read_smb(mesh_file);
+  const Smb s = read_mesh(mesh_file);
   const Model model = read_dmg(geom_file);
   const std::vector<AssocSet> sets = read_assoc(assoc_file, model);
   for (int d = 0; d < 4; ++d)

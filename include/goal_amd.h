@@ -114,8 +114,10 @@ int goal_disc_create_box(int nx, int ny, int nz, double lx, double ly, double lz
 /* goal's mesh inputs, read on the host: apf::loadMdsMesh(geom file, mesh
  * file) + read_sets(assoc file) (src/goal_disc.cpp:28-99), then the sets of
  * compute_elem_sets / compute_side_sets / compute_node_sets
- * (src/goal_disc.cpp:347-396).  mesh_file: a one-part SCOREC MDS file of
- * version 5 holding tetrahedra (the reference's *-serial0.smb meshes);
+ * (src/goal_disc.cpp:347-396).  mesh_file: a SCOREC MDS mesh of version 5
+ * holding tetrahedra -- a one-part file (the reference's *-serial0.smb), or
+ * "name.smb" for a mesh stored in parts name0.smb, name1.smb, ..., which are
+ * merged through their shared vertices;
  * geom_file: its gmi .dmg model; assoc_file: "node set / side set / elem set"
  * lines with (model dim, model tag) entries.  Node sets hold the nodes
  * classified in the closure of their model entities; side sets the triangles

@@ -131,14 +131,15 @@ def test_cpp_surface_known_answer_and_dwr_with_traction(case):
             assert abs(gpu[k]) < 1e-11, k
 
 
+@pytest.mark.parametrize("mesh", ["cube-serial0.smb", "cube.smb"])
 @pytest.mark.parametrize("case", sorted(drivers.KNOWN))
-def test_gpu_known_answer_on_reference_mesh_files(case):
+def test_gpu_known_answer_on_reference_mesh_files(case, mesh):
     """The regression inputs end to end as goal runs them: the reference's
     cube mesh, model and association file read by the library, a Disc with
     the file's node sets and side set, device Newton.  The final avg disp is
     the stored value."""
     from tests.test_known_answers import _cube_file
-    disc = _cube_file(1).disc()
+    disc = _cube_file(1, mesh).disc()
     gpu = drivers.gpu_known_newton(case, disc=disc)
     print(case, gpu["iters"], gpu["J"] - drivers.KNOWN[case]["value"])
     assert abs(gpu["J"] - drivers.KNOWN[case]["value"]) < 1e-14

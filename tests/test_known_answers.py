@@ -115,9 +115,11 @@ def test_boundary_faces_of_box_face():
 GOLDEN_SMB = __import__("pathlib").Path(__file__).resolve().parent / "golden" / "smb"
 
 
-def _cube_file(order=1):
+def _cube_file(order=1, mesh="cube-serial0.smb"):
+    """The reference's cube: "cube-serial0.smb" (one part) or "cube.smb" (its
+    four parts cube0.smb ... cube3.smb, as the example inputs name it)."""
     import goal_amd as g
-    return g.MeshFile(GOLDEN_SMB / "cube.dmg", GOLDEN_SMB / "cube-serial0.smb", GOLDEN_SMB / "cube.txt", order=order)
+    return g.MeshFile(GOLDEN_SMB / "cube.dmg", GOLDEN_SMB / mesh, GOLDEN_SMB / "cube.txt", order=order)
 
 
 def faces_as_nodes(conn, order, elems, faces):
@@ -181,12 +183,32 @@ def test_mesh_file_reader_rejects_bad_input(tmp_path):
         g.MeshFile(GOLDEN_SMB / "cube.dmg", GOLDEN_SMB / "cube-serial0.smb", tmp_path / "bad.txt")
 
 
+def test_partitioned_mesh_merges_to_the_serial_one():
+    """cube.smb as four parts (cube0..3.smb, joined through their shared
+    vertices) is the one-part cube-serial0.smb: the same vertices, tets,
+    node sets and side-set sizes; P2 edge nodes are not duplicated."""
+    def key(c):
+        return sorted(map(tuple, np.round(c, 12).tolist()))
+
+    s, p = _cube_file(1), _cube_file(1, "cube.smb")
+    cs, ks, _ = s.mesh()
+    cp, kp, _ = p.mesh()
+    assert (p.num_nodes, p.num_elems) == (51, 132) and key(cs) == key(cp)
+    assert sorted(tuple(key(cs[t])) for t in ks) == sorted(tuple(key(cp[t])) for t in kp)
+    for name, comp, val in (("xmin", 0, 0.0), ("xmax", 0, 1.0), ("ymin", 1, 0.0), ("zmin", 2, 0.0)):
+        assert np.array_equal(np.sort(p.node_set(name)), np.nonzero(cp[:, comp] == val)[0])
+    assert len(p.side_set("ymax")[0]) == len(s.side_set("ymax")[0]) == 16
+    assert _cube_file(2, "cube.smb").num_nodes == 51 + 230
+
+
+@pytest.mark.parametrize("mesh", ["cube-serial0.smb", "cube.smb"])
 @pytest.mark.parametrize("case", sorted(drivers.KNOWN))
-def test_oracle_reproduces_known_answer_on_reference_mesh(case):
-    """The regression inputs as the reference runs them: its cube mesh, model
-    and association file (read by the library), its node sets and side set,
-    P1 with one point.  The oracle's Newton gives the stored values."""
-    f = _cube_file(1)
+def test_oracle_reproduces_known_answer_on_reference_mesh(case, mesh):
+    """The regression inputs as the reference runs them: its cube mesh (one
+    part, or the four parts the example inputs name), model and association
+    file (read by the library), its node sets and side set, P1 with one
+    point.  The oracle's Newton gives the stored values."""
+    f = _cube_file(1, mesh)
     coords, conn, _ = f.mesh()
     sets = {k: f.node_set(k) for k in ("xmin", "ymin", "zmin", "xmax")}
     ymax = faces_as_nodes(conn, 1, *f.side_set("ymax"))
