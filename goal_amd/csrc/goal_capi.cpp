@@ -2101,7 +2101,39 @@ int goal_nested_create(goal_disc base, goal_nested* out) {
     n->edge_parents_host = r.edge_parents;
     n->parent.upload(r.parent.data(), (int64_t)r.parent.size());
     n->edge_parents.upload(r.edge_parents.data(), (int64_t)r.edge_parents.size());
+    // given side sets: each base face splits into four faces of its children
+    std::map<std::string, std::vector<int32_t>> fine_sides;
+    if (!base->side_sets.empty()) {
+      std::map<std::pair<int32_t, int32_t>, int32_t> mid;
+      for (int64_t k = 0; k < n->num_new; ++k) {
+        const int32_t a = r.edge_parents[2 * k], b = r.edge_parents[2 * k + 1];
+        mid[{std::min(a, b), std::max(a, b)}] = (int32_t)(n->num_old + k);
+      }
+      auto m = [&](int32_t a, int32_t b) { return mid.at({std::min(a, b), std::max(a, b)}); };
+      for (auto const& kv : base->side_sets) {
+        std::vector<int32_t>& out = fine_sides[kv.first];
+        for (int32_t side : kv.second) {
+          const int64_t e = side / 4;
+          const int32_t* c = &base->mesh.conn[e * 4];
+          const int32_t a = c[kTetFace[side % 4][0]], b = c[kTetFace[side % 4][1]], d = c[kTetFace[side % 4][2]];
+          std::array<std::array<int32_t, 3>, 4> sub = {{{a, m(a, b), m(d, a)}, {b, m(a, b), m(b, d)},
+                                                        {d, m(b, d), m(d, a)}, {m(a, b), m(b, d), m(d, a)}}};
+          for (auto& t : sub) std::sort(t.begin(), t.end());
+          for (int64_t ch = 8 * e; ch < 8 * e + 8; ++ch) {
+            const int32_t* cc = &r.fine.conn[ch * 4];
+            for (int f = 0; f < 4; ++f) {
+              std::array<int32_t, 3> t = {cc[kTetFace[f][0]], cc[kTetFace[f][1]], cc[kTetFace[f][2]]};
+              std::sort(t.begin(), t.end());
+              if (std::find(sub.begin(), sub.end(), t) != sub.end()) out.push_back((int32_t)(4 * ch + f));
+            }
+          }
+        }
+        std::sort(out.begin(), out.end());
+        require(out.size() == 4 * kv.second.size(), "nested refinement: a side did not split into four");
+      }
+    }
     n->fine = new_disc(std::move(r.fine));
+    n->fine->side_sets = std::move(fine_sides);
     *out = n.release();
     return GOAL_OK;
   });

@@ -67,6 +67,19 @@ class Disc {
     check(goal_disc_create_box(nx, ny, nz, lx, ly, lz, order, jitter, seed, &h));
     return std::unique_ptr<Disc>(new Disc(h, true));
   }
+  // Disc(ParameterList) with "geom file", "mesh file", "assoc file"
+  // (src/goal_disc.cpp:101-120): goal's mesh inputs with their node, side
+  // and element sets; order 2 adds edge-midpoint nodes.
+  static std::unique_ptr<Disc> load(std::string const& geom_file, std::string const& mesh_file,
+                                    std::string const& assoc_file, int order = 1) {
+    goal_mesh_file f = nullptr;
+    check(goal_mesh_file_read(geom_file.c_str(), mesh_file.c_str(), assoc_file.c_str(), order, &f));
+    goal_disc h = nullptr;
+    const int st = goal_disc_create_from_file(f, &h);
+    goal_mesh_file_destroy(f);
+    check(st);
+    return std::unique_ptr<Disc>(new Disc(h, true));
+  }
   ~Disc() {
     if (own_ && h_) goal_disc_destroy(h_);
   }

@@ -17,7 +17,8 @@
 //                    (example/primal/neohookean_uniaxial_3D.yaml), j2_uniaxial
 //                    (J2_uniaxial_3D.yaml) or j2_traction (J2_traction_3D.yaml,
 //                    a traction 1.0*t in y on ymax), in place of the default
-//                    zmin/zmax compression
+//                    zmin/zmax compression; known:<case>:<dir> runs it on the
+//                    mesh files <dir>/cube.dmg, <dir>/cube.smb, <dir>/cube.txt
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -77,9 +78,17 @@ int main(int argc, char** argv) {
   const int n = std::atoi(argv[4]), steps = std::atoi(argv[5]);
   const int tangent = argc > 6 ? std::atoi(argv[6]) : 64;
   const bool gmres = argc > 7 && std::string(argv[7]) == "gmres";
-  const std::string extra = argc > 8 ? argv[8] : "";
+  std::string extra = argc > 8 ? argv[8] : "";
   try {
-    auto disc = goal::Disc::box(n, n, n, 1.0, 1.0, 1.0, order, 0.1, 11);
+    std::unique_ptr<goal::Disc> disc;
+    const size_t files = extra.rfind("known:", 0) == 0 ? extra.find(':', 6) : std::string::npos;
+    if (files != std::string::npos) {
+      const std::string dir = extra.substr(files + 1);
+      disc = goal::Disc::load(dir + "/cube.dmg", dir + "/cube.smb", dir + "/cube.txt", order);
+      extra = extra.substr(0, files);
+    } else {
+      disc = goal::Disc::box(n, n, n, 1.0, 1.0, 1.0, order, 0.1, 11);
+    }
     goal::MechanicsParams mp;
     mp.model = model == 2 ? "J2" : model == 0 ? "elastic" : "neohookean";
     mp.quad_order = quad;

@@ -194,9 +194,12 @@ def known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, dwr=False, mes
         if step < KNOWN_STEPS:
             st.update()
     out = {"iters": iters, "J": J, "norm": norm}
-    if dwr and order == 1 and mesh is None:
+    if dwr and order == 1:
+        # on the nested mesh the sets are the unit cube's faces, found from the
+        # coordinates (a mesh file's sets are exactly those faces)
         tbc = (lambda c: face_nodes(c, 1, 1.0), known_traction) if k["traction"] else None
-        out.update(nested_estimate(coords, conn, x, st, model, mats, quad, float(KNOWN_STEPS), bcs=bcs, tbc=tbc))
+        out.update(nested_estimate(coords, conn, x, st, model, mats, quad, float(KNOWN_STEPS), bcs=known_bcs(case),
+                                   tbc=tbc))
     return out
 
 

@@ -143,3 +143,27 @@ def test_gpu_known_answer_on_reference_mesh_files(case, mesh):
     gpu = drivers.gpu_known_newton(case, disc=disc)
     print(case, gpu["iters"], gpu["J"] - drivers.KNOWN[case]["value"])
     assert abs(gpu["J"] - drivers.KNOWN[case]["value"]) < 1e-14
+
+
+@pytest.mark.parametrize("case", sorted(drivers.KNOWN))
+def test_cpp_surface_known_answer_on_reference_mesh_files(case):
+    """goal::Disc::load (C++ surface) on the reference's four-part cube, then
+    goal::Primal with its Dirichlet and traction entries: the stored value.
+    goal::NestedAdjoint then refines the file's side set with the mesh, and
+    its estimates match the oracle's on the same mesh (as in the box test
+    above)."""
+    from tests.test_known_answers import GOLDEN_SMB, _cube_file, faces_as_nodes
+    gpu = run_surface(1, 1, drivers.KNOWN[case]["model"], 3, drivers.KNOWN_STEPS, extra=f"known:{case}:{GOLDEN_SMB}")
+    f = _cube_file(1, "cube.smb")
+    coords, conn, _ = f.mesh()
+    sets = {k: f.node_set(k) for k in ("xmin", "ymin", "zmin", "xmax")}
+    ref = drivers.known_newton(case, mesh=(coords, conn, sets, faces_as_nodes(conn, 1, *f.side_set("ymax"))),
+                               dwr=True)
+    print(case, gpu, ref)
+    assert abs(gpu["J"] - drivers.KNOWN[case]["value"]) < 1e-14
+    assert gpu["iters"] == ref["iters"]
+    for k in ("eta_sum", "bound", "zR"):
+        if abs(ref[k]) > 1e-12:
+            assert abs(gpu[k] - ref[k]) <= 1e-8 * abs(ref[k]), k
+        else:
+            assert abs(gpu[k]) < 1e-11, k
