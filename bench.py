@@ -161,7 +161,32 @@ def cpu_baseline(n_sample: int, threads: int, wl, reps: int = 5):
         if k > 0:
             times.append(dt)
     med = statistics.median(times)
-    return m.num_elems / med, m.num_elems, med
+    return m.num_elems / med, m.num_elems, med, (coords, conn, x, R, V)
+
+
+def accuracy(g, wl, sample):
+    """The GPU assembly of the CPU-baseline sample against the oracle's R and
+    dR/du of the same run (normwise relative), FP64 and FP32-tangent
+    element matrices."""
+    import numpy as np
+
+    coords, conn, x, R, V = sample
+    disc = g.Disc.from_arrays(wl["order"], coords, conn)
+    st = g.States(disc, wl["model"], wl["quad"])
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=wl["model"], **wl["mat"])], stabilization=True,
+                       quad_order=wl["quad"])
+    disc.set_soln(x)
+    si = g.SolInfo(disc)
+    out = {}
+    for tangent in (64, 32):
+        mech.set_option("tangent", tangent)
+        si.zero_all()
+        mech.jacobian(si, adjoint=False, save_states=False)
+        r, v = si.R(), si.vals()
+        tag = "" if tangent == 64 else "_fp32_tangent"
+        out["R_rel" + tag] = float(np.linalg.norm(r - R) / np.linalg.norm(R))
+        out["dRdu_rel" + tag] = float(np.linalg.norm(v - V) / np.linalg.norm(V))
+    return out
 
 
 def _read_csv(path):
@@ -773,10 +798,16 @@ def main() -> int:
     }
 
     cpu = None
+    acc = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {args.cpu_sample}^3-hex sample, {threads} threads")
-        v, ne_s, med = cpu_baseline(args.cpu_sample, threads, wl)
+        v, ne_s, med, sample = cpu_baseline(args.cpu_sample, threads, wl)
+        acc = accuracy(g, wl, sample)
+        acc["bar_fp64"] = 1e-11
+        acc["sample"] = "the cpu_baseline sample, GPU against the oracle's output of the same run"
+        log(f"accuracy on the sample: R {acc['R_rel']:.2e}, dR/du {acc['dRdu_rel']:.2e} (FP64); "
+            f"dR/du {acc['dRdu_rel_fp32_tangent']:.2e} (FP32 tangent)")
         cpu = {
             "value": round(v, 1),
             "unit": "elem/s",
@@ -824,6 +855,7 @@ def main() -> int:
             "traffic_source": traffic_src,
             "tangent_bits": args.tangent,
             "iterate": iterate,
+            "accuracy": acc,
             "chunks": chunks(args.tangent),
             "chunked_split": "with chunks > 1 the element time runs from the first chunk to the last and "
                              "overlaps the side-stream gathers of finished node runs; the gather time is "
