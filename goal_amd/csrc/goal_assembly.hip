@@ -125,8 +125,24 @@ struct SLoc {
 };
 struct SNone {};
 
-template <int NN, int NQ, int MODEL, int KIND, int G>
-struct Smem {
+// neo-Hookean: the next group's phase-A data, fetched by wave 1 while wave 0
+// computes the point states (phase C1)
+template <int NN, int G>
+struct SPrefetch {
+  int32_t pconn[G][NN];
+  int32_t prec[G][NN];
+  double pX[G][4][3];
+  double px[G][4 * NN];
+};
+struct SNoPrefetch {};
+// (FP64 element matrices and residuals: the FP32-tangent kernels run chunked
+// beside the gather, where it measured slightly slower, 10.50 -> 10.55 ms)
+template <int NN, int NQ, int MODEL, int KIND, typename TE>
+constexpr bool kPrefetch = MODEL == MODEL_NEOHOOKEAN && KIND != ASM_LOCALIZE && Layout<NN, NQ, MODEL>::T >= 128 &&
+                           std::is_same<TE, double>::value;
+
+template <int NN, int NQ, int MODEL, int KIND, int G, typename TE = double>
+struct Smem : std::conditional<kPrefetch<NN, NQ, MODEL, KIND, TE>, SPrefetch<NN, G>, SNoPrefetch>::type {
   using Ext = typename std::conditional<
       KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT, SJac<NQ, MODEL, G>,
       typename std::conditional<KIND == ASM_LOCALIZE, SLoc<NN, NQ, G>, SNone>::type>::type;
@@ -251,7 +267,7 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
   constexpr int G = Lay::G, ND = Lay::ND, T = Lay::T;
   constexpr bool JAC = (KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT);
   constexpr bool LOC = (KIND == ASM_LOCALIZE);
-  __shared__ Smem<NN, NQ, MODEL, KIND, G> sm;
+  __shared__ Smem<NN, NQ, MODEL, KIND, G, TE> sm;
   auto& s = sm.b;
   const int tid = threadIdx.x;
   const int64_t ne = a.num_elems;
@@ -263,6 +279,7 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     }
   }
   const int64_t nqtot = ne * NQ;
+  bool fetched = false;  // this group's phase-A data is in the prefetch buffers
 
   for (int64_t it = 0;; ++it) {
     const int64_t grp0 = group_of(it, ngroups, a.xcd_remap);
@@ -272,6 +289,18 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     const int nv = (int)min((int64_t)G, ne - e0);  // elements of this group
 
     // ---- A: gather (Displacement/Pressure::gather, src/goal_displacement.cpp:139-156)
+    constexpr bool PF = kPrefetch<NN, NQ, MODEL, KIND, TE>;
+    if (PF && fetched) {
+      // this group's data came in during the previous group's phase C1
+      if constexpr (PF) {
+        for (int t = tid; t < nv * NN; t += T) {
+          s.conn[t / NN][t % NN] = sm.pconn[t / NN][t % NN];
+          s.rec[t / NN][t % NN] = sm.prec[t / NN][t % NN];
+        }
+        for (int t = tid; t < nv * 12; t += T) s.X[t / 12][(t % 12) / 3][t % 3] = sm.pX[t / 12][(t % 12) / 3][t % 3];
+        for (int t = tid; t < nv * ND; t += T) s.x[t / ND][t % ND] = sm.px[t / ND][t % ND];
+      }
+    } else {
     for (int t = tid; t < nv * NN; t += T) {
       s.conn[t / NN][t % NN] = a.conn[e0 * NN + t];
       s.rec[t / NN][t % NN] = a.rec[e0 * NN + t];
@@ -289,6 +318,7 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     }
     if constexpr (LOC)
       for (int t = tid; t < nv * NN; t += T) sm.ext.zc[t / NN][t % NN] = a.z_coarse[4 * (int64_t)s.conn[t / NN][t % NN] + 3];
+    }
     __syncthreads();
 
     // ---- B: shape functions at the quadrature points (u/uw/p/pw at_point)
@@ -362,6 +392,27 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     __syncthreads();
 
     // ---- C1: quadrature-point state (Kinematics, model, Mixed; states), one lane per point
+    if constexpr (PF) {
+      // wave 1 idles in this phase (the points take fewer lanes than one
+      // wave): it fetches the next group's connectivity, incidence records,
+      // vertex coordinates and DOFs, one (element, node) per lane
+      static_assert(G * NQ <= 64 && G * NN <= 64, "phase C1 on wave 0, the next group's nodes on wave 1");
+      const int64_t gn = group_of(it + 1, ngroups, a.xcd_remap);
+      fetched = gn >= 0;
+      const int l = tid - 64;
+      if (fetched && l >= 0 && l < 64) {
+        const int64_t en = (g_lo + gn) * G;
+        if (l < (int)min((int64_t)G, ne - en) * NN) {
+          const int el = l / NN, n = l % NN;
+          const int32_t c = a.conn[en * NN + l];
+          sm.pconn[el][n] = c;
+          sm.prec[el][n] = a.rec[en * NN + l];
+          if (n < 4)
+            for (int k = 0; k < 3; ++k) sm.pX[el][n][k] = a.coords[3 * (int64_t)c + k];
+          for (int k = 0; k < 4; ++k) sm.px[el][4 * n + k] = a.x[4 * (int64_t)c + k];
+        }
+      }
+    }
     for (int t = tid; t < nv * NQ; t += T) {
       const int el = t / NQ, q = t % NQ;
       const int64_t e = e0 + el;
