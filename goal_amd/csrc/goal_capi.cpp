@@ -695,6 +695,7 @@ struct goal_adapt_s {
   goal::Adaptation a;
 };
 
+extern "C++" {
 namespace {
 goal::HostMesh linear_mesh(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn) {
   goal::HostMesh m;
@@ -708,6 +709,7 @@ goal::HostMesh linear_mesh(int64_t num_nodes, const double* coords, int64_t num_
   return m;
 }
 }  // namespace
+}  // extern "C++"
 
 int goal_size_field(int64_t num_nodes, const double* coords, int64_t num_elems, const int32_t* conn,
                     const double* eta, int64_t target, int p_order, double* vtx_size) {
@@ -1016,6 +1018,7 @@ int goal_disc_get_node_set(goal_disc d, const char* name, int32_t* nodes) {
 }
 
 // ------------------------------------------------------------ side sets
+extern "C++" {
 namespace {
 
 // Local vertices of tet face f, in apf's order.
@@ -1065,6 +1068,7 @@ const std::vector<int32_t>& side_set(goal_disc d, const char* name) {
 }
 
 }  // namespace
+}  // extern "C++"
 
 int goal_disc_add_side_set(goal_disc d, const char* name, int64_t n, const int32_t* elems, const int32_t* faces) {
   return guard([&]() -> int {
