@@ -2092,11 +2092,78 @@ int goal_tbc_apply(goal_disc d, goal_solinfo s, const char* side_set_name, const
 }
 
 // ------------------------------------------------------------ nested DWR
+// Nested::Nested's uniform refinement (src/goal_nested.cpp:13-28, 144-154) on
+// the device: the children, the new vertices' parent pairs and coordinates
+// come from kernels over the base Disc's device mesh and CSR graph, numbered
+// as goal::refine_uniform numbers them (edges in (lo, hi) order), so the same
+// fine mesh results; the host then adds the node sets and builds the fine
+// Disc's graph.
+extern "C++" {
+namespace {
+goal::Refinement refine_on_device(goal_disc base) {
+  const goal::HostMesh& m = base->mesh;
+  const int64_t nv = m.num_nodes, ne = m.num_elems;
+  DevBuf<int64_t> ud;
+  ud.alloc(nv);
+  HIPCHK(goal::launch_upper_degree(base->row_ptr.p, base->cols.p, nv, ud.p, stream()));
+  std::vector<int64_t> eb(nv + 1, 0);
+  ud.download(eb.data() + 1, nv);
+  for (int64_t a = 0; a < nv; ++a) eb[a + 1] += eb[a];
+  const int64_t nedges = eb[nv];
+  require(nv + nedges <= (int64_t)INT32_MAX / goal::kNumEqs, "nested refinement: mesh too large");
+  DevBuf<int64_t> ebase;
+  ebase.upload(eb.data(), nv + 1);
+  DevBuf<double> fX;
+  fX.alloc(3 * (nv + nedges));
+  HIPCHK(goal::launch_copy(fX.p, base->coords.p, 3 * nv, stream()));
+  DevBuf<int32_t> ep, fconn, status;
+  ep.alloc(2 * nedges);
+  fconn.alloc(32 * ne);
+  const int32_t zero = 0;
+  status.upload(&zero, 1);
+  HIPCHK(goal::launch_refine(base->conn.p, base->coords.p, base->row_ptr.p, base->cols.p, ebase.p, ne, nv, ep.p, fX.p,
+                             fconn.p, status.p, stream()));
+  int32_t st = 0;
+  status.download(&st, 1);
+  require(st == 0, "nested refinement: an element edge is missing from the base graph");
+  goal::Refinement r;
+  r.num_old_vertices = nv;
+  r.edge_parents.resize(2 * nedges);
+  ep.download(r.edge_parents.data(), 2 * nedges);
+  r.parent.resize(8 * ne);
+  for (int64_t c = 0; c < 8 * ne; ++c) r.parent[c] = (int32_t)(c / 8);
+  goal::HostMesh& f = r.fine;
+  f.order = 1;
+  f.nodes_per_elem = 4;
+  f.num_nodes = nv + nedges;
+  f.num_elems = 8 * ne;
+  f.coords.resize(3 * f.num_nodes);
+  fX.download(f.coords.data(), 3 * f.num_nodes);
+  f.conn.resize(32 * ne);
+  fconn.download(f.conn.data(), 32 * ne);
+  f.elem_set.resize(8 * ne);
+  for (int64_t c = 0; c < 8 * ne; ++c) f.elem_set[c] = m.elem_set.empty() ? 0 : m.elem_set[c / 8];
+  f.num_elem_sets = m.num_elem_sets;
+  // node sets: old members plus new vertices whose two parents are members
+  for (auto const& kv : m.node_sets) {
+    std::vector<char> in(nv, 0);
+    for (int32_t n : kv.second) in[n] = 1;
+    std::vector<int32_t> set(kv.second);
+    for (int64_t k = 0; k < nedges; ++k)
+      if (in[r.edge_parents[2 * k]] && in[r.edge_parents[2 * k + 1]]) set.push_back((int32_t)(nv + k));
+    f.node_sets[kv.first] = std::move(set);
+  }
+  goal::finalize_mesh(f);
+  return r;
+}
+}  // namespace
+}  // extern "C++"
+
 int goal_nested_create(goal_disc base, goal_nested* out) {
   return guard([&]() -> int {
     require(base && out, "goal_nested_create: bad arguments");
     require(base->mesh.order == 1, "nested refinement needs a linear base mesh");
-    goal::Refinement r = goal::refine_uniform(base->mesh);
+    goal::Refinement r = refine_on_device(base);
     std::unique_ptr<goal_nested_s> n(new goal_nested_s);
     n->base = base;
     n->num_old = r.num_old_vertices;

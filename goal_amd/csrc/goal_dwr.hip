@@ -120,6 +120,120 @@ hipError_t launch_transfer_state(const double* base, double* fine, int ncomp, in
   return hipGetLastError();
 }
 
+// ---- uniform refinement on the device (the nested mesh, 8 children per tet)
+// Edge numbering: the edges (lo, hi), lo < hi, in lexicographic order, as the
+// host refinement numbers them.  With the base mesh's sorted node adjacency
+// (the DOF CSR: row 4a holds 4b for every neighbour b of a, ascending), edge
+// (lo, hi) is number base[lo] + (rank of hi among the neighbours above lo),
+// base[] being the exclusive prefix sum of the neighbour counts above each node.
+__device__ inline int64_t upper_begin(const int64_t* row_ptr, const int32_t* cols, int64_t a, int64_t* deg) {
+  const int64_t b0 = row_ptr[4 * a];
+  *deg = (row_ptr[4 * a + 1] - b0) / 4;
+  // first neighbour above a (a itself is a neighbour)
+  int64_t lo = 0, hi = *deg;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (cols[b0 + 4 * mid] <= 4 * a) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void upper_degree_kernel(const int64_t* row_ptr, const int32_t* cols, int64_t nv,
+                                                              int64_t* ud) {
+  for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < nv; a += (int64_t)gridDim.x * kBlock) {
+    int64_t deg;
+    const int64_t k = upper_begin(row_ptr, cols, a, &deg);
+    ud[a] = deg - k;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void refine_edges_kernel(const int64_t* row_ptr, const int32_t* cols,
+                                                              const int64_t* ebase, const double* X, int64_t nv,
+                                                              int32_t* edge_parents, double* fX) {
+  for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < nv; a += (int64_t)gridDim.x * kBlock) {
+    int64_t deg;
+    const int64_t k0 = upper_begin(row_ptr, cols, a, &deg);
+    const int64_t b0 = row_ptr[4 * a];
+    for (int64_t k = k0; k < deg; ++k) {
+      const int64_t b = cols[b0 + 4 * k] / 4, id = ebase[a] + (k - k0);
+      edge_parents[2 * id] = (int32_t)a;
+      edge_parents[2 * id + 1] = (int32_t)b;
+      for (int c = 0; c < 3; ++c) fX[3 * (nv + id) + c] = 0.5 * (X[3 * a + c] + X[3 * b + c]);
+    }
+  }
+}
+
+__device__ inline double tet_det4(const double* x0, const double* x1, const double* x2, const double* x3) {
+  double u[3], v[3], w[3];
+  for (int i = 0; i < 3; ++i) {
+    u[i] = x1[i] - x0[i];
+    v[i] = x2[i] - x0[i];
+    w[i] = x3[i] - x0[i];
+  }
+  return u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) + u[2] * (v[0] * w[1] - v[1] * w[0]);
+}
+
+__global__ __launch_bounds__(kBlock) void refine_elems_kernel(const int32_t* conn, const int64_t* row_ptr,
+                                                              const int32_t* cols, const int64_t* ebase,
+                                                              const double* fX, int64_t ne, int64_t nv,
+                                                              int32_t* fconn, int32_t* status) {
+  constexpr int E[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};  // edges 01,12,20,03,13,23
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+    int32_t v[4], m[6];
+    for (int k = 0; k < 4; ++k) v[k] = conn[4 * e + k];
+    for (int ed = 0; ed < 6; ++ed) {
+      const int32_t p = v[E[ed][0]], q = v[E[ed][1]];
+      const int64_t lo = p < q ? p : q, hi = p < q ? q : p;
+      int64_t deg;
+      const int64_t k0 = upper_begin(row_ptr, cols, lo, &deg);
+      const int64_t b0 = row_ptr[4 * lo];
+      int64_t l = k0, h = deg;  // hi is among the neighbours above lo
+      while (l < h) {
+        const int64_t mid = (l + h) / 2;
+        if (cols[b0 + 4 * mid] < 4 * hi) l = mid + 1;
+        else h = mid;
+      }
+      if (l >= deg || cols[b0 + 4 * l] != 4 * hi) {  // not an edge of the graph: report, stay in bounds
+        atomicOr(status, 1);
+        l = k0;
+      }
+      m[ed] = (int32_t)(nv + ebase[lo] + (l - k0));
+    }
+    const int32_t kids[8][4] = {{v[0], m[0], m[2], m[3]}, {m[0], v[1], m[1], m[4]}, {m[2], m[1], v[2], m[5]},
+                                {m[3], m[4], m[5], v[3]}, {m[0], m[2], m[3], m[4]}, {m[0], m[2], m[1], m[4]},
+                                {m[2], m[3], m[4], m[5]}, {m[2], m[1], m[4], m[5]}};
+    for (int c = 0; c < 8; ++c) {
+      int32_t cc[4] = {kids[c][0], kids[c][1], kids[c][2], kids[c][3]};
+      const double d = tet_det4(&fX[3 * (int64_t)cc[0]], &fX[3 * (int64_t)cc[1]], &fX[3 * (int64_t)cc[2]],
+                                &fX[3 * (int64_t)cc[3]]);
+      if (d < 0) {
+        const int32_t t = cc[2];
+        cc[2] = cc[3];
+        cc[3] = t;
+      }
+      for (int k = 0; k < 4; ++k) fconn[4 * (8 * e + c) + k] = cc[k];
+    }
+  }
+}
+
+hipError_t launch_upper_degree(const int64_t* row_ptr, const int32_t* cols, int64_t nv, int64_t* ud, hipStream_t s) {
+  if (nv <= 0) return hipSuccess;
+  hipLaunchKernelGGL(upper_degree_kernel, dim3(grid_for(nv)), dim3(kBlock), 0, s, row_ptr, cols, nv, ud);
+  return hipGetLastError();
+}
+
+hipError_t launch_refine(const int32_t* conn, const double* X, const int64_t* row_ptr, const int32_t* cols,
+                         const int64_t* ebase, int64_t ne, int64_t nv, int32_t* edge_parents, double* fX,
+                         int32_t* fconn, int32_t* status, hipStream_t s) {
+  if (nv <= 0 || ne <= 0) return hipSuccess;
+  hipLaunchKernelGGL(refine_edges_kernel, dim3(grid_for(nv)), dim3(kBlock), 0, s, row_ptr, cols, ebase, X, nv,
+                     edge_parents, fX);
+  hipLaunchKernelGGL(refine_elems_kernel, dim3(grid_for(ne)), dim3(kBlock), 0, s, conn, row_ptr, cols, ebase, fX,
+                     ne, nv, fconn, status);
+  return hipGetLastError();
+}
+
 hipError_t launch_set_coarse(double* z, const int32_t* edge_parents, int64_t num_old, int64_t num_new,
                              hipStream_t s) {
   if (num_new <= 0) return hipSuccess;

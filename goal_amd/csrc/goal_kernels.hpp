@@ -149,6 +149,16 @@ hipError_t launch_take_rows(const int64_t* rows, const double* src, int64_t n, d
 // DWR (goal_dwr.hip)
 hipError_t launch_set_coarse(double* z, const int32_t* edge_parents, int64_t num_old, int64_t num_new,
                              hipStream_t s);
+// Uniform refinement (8 children per tet) from the base mesh's DOF CSR graph:
+// ud[a] = number of neighbours of node a above a; with ebase = its exclusive
+// prefix sum (ebase[nv] = number of edges), launch_refine writes the edges'
+// parent pairs and midpoints (fX rows nv..) and the children (fconn [8 ne][4],
+// positively oriented; fX rows 0..nv-1 must hold the base coordinates).
+// status |= 1 if an element edge is missing from the graph.
+hipError_t launch_upper_degree(const int64_t* row_ptr, const int32_t* cols, int64_t nv, int64_t* ud, hipStream_t s);
+hipError_t launch_refine(const int32_t* conn, const double* X, const int64_t* row_ptr, const int32_t* cols,
+                         const int64_t* ebase, int64_t ne, int64_t nv, int32_t* edge_parents, double* fX,
+                         int32_t* fconn, int32_t* status, hipStream_t s);
 hipError_t launch_subtract(const double* a, const double* b, double* out, int64_t n, hipStream_t s);
 hipError_t launch_compute_error(int order, const int32_t* conn, int64_t ne, const double* e,
                                 double* eta, hipStream_t s);

@@ -157,6 +157,31 @@ def test_dirichlet_rows(order):
     assert np.allclose(coords[zmax, 2], 1.0) and np.allclose(coords[zmin, 2], 0.0)
 
 
+@pytest.mark.parametrize("dims,seed", [((2, 2, 2), 5), ((7, 3, 5), 11), ((12, 12, 12), 2)])
+def test_nested_refinement_on_device_equals_host(dims, seed):
+    """The nested Disc's uniform refinement runs on the device (children,
+    parent pairs, midpoints from the base Disc's CSR graph).  It is the host
+    refinement bit for bit: coordinates, children and their orientation,
+    parent maps, node sets."""
+    coords, conn = g.mesh_box(*dims, order=1, jitter=0.1, seed=seed)
+    base = g.Disc.from_arrays(1, coords, conn)
+    for name in ("zmin", "xmax"):
+        comp, val = (2, 0.0) if name == "zmin" else (0, 1.0)
+        base.add_node_set(name, np.nonzero(coords[:, comp] == val)[0])
+    nested = g.Nested(base)
+    hc, hconn, hparent, hep = g.mesh_refine(coords, conn)
+    fc, fconn, _ = nested.disc.mesh()
+    parent, ep = nested.maps()
+    assert np.array_equal(fc, hc) and np.array_equal(fconn, hconn)
+    assert np.array_equal(parent, hparent) and np.array_equal(np.ravel(ep), np.ravel(hep))
+    nv = coords.shape[0]
+    pairs = np.reshape(hep, (-1, 2))
+    for name in ("zmin", "xmax"):
+        b = set(base.node_set(name).tolist())
+        want = sorted(b | {nv + k for k, (i, j) in enumerate(pairs.tolist()) if i in b and j in b})
+        assert sorted(nested.disc.node_set(name).tolist()) == want, name
+
+
 @pytest.mark.parametrize("model", [1, 2])
 def test_localize_and_indicators(model):
     base = g.Disc.box(2, 2, 2, order=1, jitter=0.1, seed=5)
