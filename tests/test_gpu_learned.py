@@ -187,7 +187,7 @@ def test_learned_falls_back_to_nested_out_of_range():
 @pytest.mark.skipif(not LP.DEFAULT_MODEL.exists(), reason="no packaged network")
 @pytest.mark.parametrize("geometry,n,seed", [("cube", 8, 31), ("bar", 4, 32)])
 def test_packaged_network_tracks_nested(geometry, n, seed):
-    """The packaged network (trained on 240 seeded runs) on unseen problems:
+    """The packaged network (trained on 260 seeded runs) on unseen problems:
     element ranking and indicator sum close to the nested DWR's, no fallback."""
     net = LP.load(LP.DEFAULT_MODEL)
     p = LP.sample_problem(np.random.default_rng(seed), geometry, n)
