@@ -97,6 +97,10 @@ Smb read_smb(const std::string& path, uint32_t part = 0) {
   s.nt = n[2];
   s.ntet = n[7];
   if (s.nv < 4 || s.ntet < 1) fail(path + ": no tetrahedra");
+  // The adjacency, coordinates and parametric coordinates follow the counts;
+  // a file too short for them is rejected before anything is sized by them.
+  const uint64_t body = 4 * (2 * (uint64_t)s.ne + 3 * (uint64_t)s.nt + 4 * (uint64_t)s.ntet) + 40 * (uint64_t)s.nv;
+  if (in.at + body > bytes.size()) fail(path + ": file ends early");
   auto down = [&](int64_t count, int per, int64_t bound, std::vector<int32_t>& out) {
     out.resize(count * per);
     for (auto& v : out) {

@@ -175,9 +175,13 @@ def test_mesh_file_reader_rejects_bad_input(tmp_path):
     bad_version = raw[:4] + (6).to_bytes(4, "big") + raw[8:]
     (tmp_path / "v6.smb").write_bytes(bad_version)
     (tmp_path / "short.smb").write_bytes(raw[:5000])
-    for name in ("v6.smb", "short.smb"):
+    # a tet count of 4e9 in the header: rejected before anything is sized by it
+    (tmp_path / "huge.smb").write_bytes(raw[:44] + (0xFFFFFFF0).to_bytes(4, "big") + raw[48:])
+    for name in ("v6.smb", "short.smb", "huge.smb"):
         with pytest.raises(g.GoalError):
             g.MeshFile(GOLDEN_SMB / "cube.dmg", tmp_path / name, GOLDEN_SMB / "cube.txt")
+    with pytest.raises(g.GoalError, match="ends early"):
+        g.MeshFile(GOLDEN_SMB / "cube.dmg", tmp_path / "huge.smb", GOLDEN_SMB / "cube.txt")
     (tmp_path / "bad.txt").write_text("node set xmin 1\n2 99\n")
     with pytest.raises(g.GoalError, match="no model entity"):
         g.MeshFile(GOLDEN_SMB / "cube.dmg", GOLDEN_SMB / "cube-serial0.smb", tmp_path / "bad.txt")
