@@ -673,7 +673,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: this build has no multi-GPU path (the assembly would shard by "
+            "bench.py measures one GPU: goal_amd has no multi-GPU path (the assembly would shard by "
             "mesh partition, one part per GPU, with an exchange of interface rows). Use --gpus 1.",
             file=sys.stderr,
         )

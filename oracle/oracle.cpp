@@ -1,6 +1,7 @@
 // ORACLE -- TEST INFRASTRUCTURE ONLY.  Loaded by tests/, by the smoke check
-// and by bench.py's cpu_baseline leg, always as the checker or the timed CPU
-// baseline, never as part of the product path.
+// and by bench.py's cpu_baseline leg, always as the reference values a result
+// is compared with or as the timed CPU baseline, never as part of the product
+// path.
 //
 // A plain C++ restatement of goal's element assembly and nested-adjoint DWR
 // path, written from the reference source as text and following its

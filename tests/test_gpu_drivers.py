@@ -68,13 +68,13 @@ def test_fp32_tangent_newton(order, quad, model, n, steps):
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
-def test_cfg1_newton_matches_oracle():
+def test_cube8_p2_newton_matches_oracle():
     """The CPU-runnable workload end to end: 8^3 hexahedra -> 3,072 P2 tets,
     neo-Hookean, 4-point quadrature, ten load steps to u_z = 0.1 on z = 1,
     Newton to ||R|| < 1e-10 with the device GMRES.  Same iteration count per
-    load step as the oracle with a direct solve (tests/golden/cfg1_newton.json),
+    load step as the oracle with a direct solve (tests/golden/cube8_p2_newton.json),
     the average displacement within 1e-9 relative at every step."""
-    ref = json.loads((GOLDEN / "cfg1_newton.json").read_text())
+    ref = json.loads((GOLDEN / "cube8_p2_newton.json").read_text())
     c = ref["config"]
     gpu = drivers.gpu_newton_steps(c["order"], c["quad"], c["model"], c["n"], c["steps"], jitter=c["jitter"],
                                    seed=c["seed"])

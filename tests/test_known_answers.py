@@ -10,7 +10,7 @@ equations of any tet mesh exactly:
   * interior and free-face nodes see P : sum_e grad N_a |e| = 0;
   * the pressure is constant, so the stabilization term vanishes;
   * avg disp integrates a linear field, which the one-point rule does exactly.
-So the reference's values hold on this build's own jittered box meshes, not only
+So the reference's values hold on goal_amd's own jittered box meshes, not only
 on its SCOREC .smb files.  The Newton iterates are homogeneous too.  Stopping
 at ||R|| < 1e-8 after the same iteration as the reference therefore gives the
 same value to rounding, and the values match to ~1e-17, far inside the
