@@ -352,7 +352,15 @@ int finish_assembly(goal_mech m) {
   return GOAL_OK;
 }
 
+// The kernels index the material table by element set: the disc's set count
+// may have grown (goal_disc_set_elem_sets) since the mechanics was created.
+void require_materials(goal_mech m) {
+  require(m->num_sets >= m->d->mesh.num_elem_sets,
+          "fewer materials than element sets (goal_disc_set_elem_sets after goal_mech_create)");
+}
+
 goal::AsmArgs asm_args(goal_mech m) {
+  require_materials(m);
   goal::AsmArgs a{};
   goal_disc d = m->d;
   a.num_elems = d->mesh.num_elems;
@@ -1652,6 +1660,9 @@ int goal_batch_tune(goal_batch b, int reps, int* blocks_out, float* ms_out) {
     };
     for (size_t i = 0; i < b->mechs.size(); ++i) {
       goal_mech m = b->mechs[i];
+      // time the path the graph captures: one pass, never chunked
+      const int overlap = m->overlap;
+      m->overlap = 0;
       // candidate persistent grids: the occupancy-sized one (0) and fixed
       // multiples of the CU count; more workgroups than element groups are
       // never useful
@@ -1669,6 +1680,7 @@ int goal_batch_tune(goal_batch b, int reps, int* blocks_out, float* ms_out) {
           best = c;
         }
       }
+      m->overlap = overlap;
       m->blocks = best;
       if (blocks_out) blocks_out[i] = best;
       if (ms_out) ms_out[i] = best_ms;
@@ -1730,6 +1742,9 @@ int goal_batch_run(goal_batch b) {
   return guard([&]() -> int {
     require(b && b->exec, "goal_batch_run: capture the batch first");
     HIPCHK(hipGraphLaunch(b->exec, stream()));
+    // the replay writes R, dMdu and the CSR values of every pair from scratch,
+    // so a zero_* still pending from before the launch is satisfied
+    for (goal_solinfo s : b->infos) s->zero_R = s->zero_dMdu = s->zero_vals = false;
     return GOAL_OK;
   });
 }
@@ -1751,6 +1766,7 @@ int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* va
     require(m && value, "goal_asm_qoi: bad arguments");
     require(kind == GOAL_QOI_AVG_DISP || kind == GOAL_QOI_AVG_VM, "unknown QoI kind");
     require(!s || s->d == m->d, "solinfo belongs to another disc");
+    require_materials(m);
     goal::QoiArgs q{};
     goal_disc d = m->d;
     q.num_elems = d->mesh.num_elems;
@@ -1782,6 +1798,7 @@ int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* va
 int goal_learn_features(goal_mech m, double qoi_x, float* feat_dev) {
   return guard([&]() -> int {
     require(m && feat_dev, "goal_learn_features: bad arguments");
+    require_materials(m);
     goal_disc d = m->d;
     const int64_t ne = d->mesh.num_elems, nn = d->mesh.num_nodes;
     // per-element integral of the von Mises stress (the AvgVM integrand over all elements)

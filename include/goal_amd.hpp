@@ -505,8 +505,7 @@ class Primal {
         compute_jacob(t_now, t_old);
         double* R = nullptr;
         sol_info_->dev(&R, nullptr, nullptr);
-        check(goal_device_copy(b, R, n));
-        scale_device(b, n, -1.0);
+        check(goal_device_scale(b, R, n, -1.0));  // b = -R on the device
         if (solve_(mech_->get_disc(), sol_info_.get(), b, du) != 0)
           throw Error(GOAL_ERR_INVALID, "linear solve failed");
         mech_->get_disc()->add_soln_dev(du);
@@ -529,12 +528,6 @@ class Primal {
   double last_norm() const { return last_norm_; }
 
  private:
-  static void scale_device(double* v, int64_t n, double a) {
-    std::vector<double> h((size_t)n);
-    check(goal_memcpy_d2h(h.data(), v, 8 * n));
-    for (auto& x : h) x *= a;
-    check(goal_memcpy_h2d(v, h.data(), 8 * n));
-  }
   PrimalParams params_;
   Mechanics* mech_;
   LinearSolve solve_;

@@ -883,6 +883,9 @@ void oracle_traction(int order, int64_t nf, const int32_t* fnodes, const double*
 }
 
 // Nested::set_coarse (in place): new vertex = mean of its two parents.
+// Pinned by tests/test_oracle.py against the base mesh's P1 interpolant
+// evaluated at each nested vertex by barycentric coordinates, and by affine
+// fields coming back unchanged.
 void oracle_set_coarse(double* z, const int32_t* edge_parents, int64_t num_old, int64_t num_new) {
   for (int64_t k = 0; k < num_new; ++k) {
     int64_t v = num_old + k, a = edge_parents[2 * k], b = edge_parents[2 * k + 1];

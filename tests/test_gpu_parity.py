@@ -301,7 +301,7 @@ def test_accumulate_without_zero():
 
 @pytest.mark.parametrize("order", [1, 2])
 def test_linear_solve(order):
-    """Device GMRES (node block-Jacobi, left preconditioned) on an assembled
+    """Device GMRES (node block-Jacobi, right preconditioned) on an assembled
     Jacobian with Dirichlet rows, against scipy's direct solve."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
@@ -332,37 +332,106 @@ def test_linear_solve(order):
     assert rel(y.numpy(), ref) < 1e-8
 
 
-def test_batch_graph_matches_eager():
-    """Three meshes (P2 neo-Hookean, P1 J2 with history, P1 neo-Hookean)
-    captured into one graph after the autotuner: every replay, with eager
-    library work and synchronizations between replays, reproduces the eager
-    assemblies bit for bit (deterministic kernels, overwrite-mode capture)."""
+def make_box(order, quad, model, dims, lengths, seed, jitter=0.1):
+    """Like make() on an nx x ny x nz box of the given side lengths."""
+    disc = g.Disc.box(*dims, lx=lengths[0], ly=lengths[1], lz=lengths[2], order=order, jitter=jitter,
+                      seed=seed)
+    m = MATS[model]
+    st = g.States(disc, model, quad)
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(**m)], stabilization=True, quad_order=quad)
+    x = random_state(disc.num_nodes, seed, scale_u=0.01)
+    disc.set_soln(x)
+    om = oracle_mesh_from(disc)
+    ost = O.States(om, model, quad)
+    if model == 2:
+        Fp, eq = random_history(disc.num_elems * st.nq, seed + 1)
+        for k, v in (("Fp_old", Fp), ("eqps_old", eq), ("Fp", Fp), ("eqps", eq)):
+            st.set(k, v)
+        ost.Fp_old[:] = Fp
+        ost.eqps_old[:] = eq
+        ost.Fp[:] = Fp
+        ost.eqps[:] = eq
+    return disc, st, mech, x, om, ost, [mat_row(m)]
+
+
+# The batched workload in miniature: meshes of uneven size and aspect ratio,
+# P2 neo-Hookean with 4 points and P1 J2 with 1 point and plastic history.
+BATCH = [
+    # (order, quad, model, (nx, ny, nz), (lx, ly, lz))
+    (2, 2, 1, (3, 3, 3), (1.0, 1.0, 1.0)),
+    (1, 1, 2, (8, 2, 2), (4.0, 1.0, 1.0)),
+    (2, 2, 1, (6, 2, 1), (6.0, 2.0, 1.0)),
+    (1, 1, 2, (5, 5, 4), (1.0, 1.0, 0.8)),
+    (2, 2, 1, (2, 2, 2), (1.0, 1.0, 1.0)),
+    (1, 1, 2, (12, 6, 6), (8.0, 2.0, 2.0)),
+    (2, 2, 1, (4, 3, 2), (2.0, 1.5, 1.0)),
+    (1, 1, 2, (3, 3, 3), (1.0, 1.0, 1.0)),
+    (2, 2, 2, (3, 2, 2), (3.0, 2.0, 2.0)),
+]
+
+
+def test_batch_graph_matches_oracle_and_eager():
+    """Nine meshes of uneven size (48 to 2,592 tets) and aspect ratio (1 to
+    6), P2 neo-Hookean 4-point, P1 J2 1-point with history and one P2 J2,
+    tuned and captured into one graph.  Every replay (with eager library work
+    and synchronizations in between, and zero_all() pending before the launch)
+    gives R, the CSR values and the updated states of every mesh within RTOL of
+    the oracle, and bit for bit the eager assemblies."""
     items = []
-    for order, quad, model, n in [(2, 2, 1, 3), (1, 1, 2, 4), (1, 1, 1, 5)]:
-        disc, st, mech, x, om, ost, mats = make(order, quad, model, n=n)
+    for k, (order, quad, model, dims, lengths) in enumerate(BATCH):
+        disc, st, mech, x, om, ost, mats = make_box(order, quad, model, dims, lengths, seed=40 + k)
         mech.set_option("check", 0)
-        items.append((disc, mech, g.SolInfo(disc)))
-    ref = []
-    for disc, mech, si in items:
-        si.zero_all()
-        mech.jacobian(si, save_states=True)
-        ref.append((si.R(), si.vals()))
-    b = g.Batch([it[1] for it in items], [it[2] for it in items])
+        items.append(dict(disc=disc, st=st, mech=mech, si=g.SolInfo(disc), om=om, ost=ost, mats=mats,
+                          model=model, quad=quad, x=x))
+    sizes = [it["disc"].num_elems for it in items]
+    assert len(items) >= 8 and max(sizes) >= 20 * min(sizes)
+    want = []
+    for it in items:
+        R, V = O.assemble(O.K_JACOBIAN, it["om"], it["model"], it["mats"], it["x"], it["ost"],
+                          quad_order=it["quad"])
+        want.append((R, V, it["ost"].sigma.copy()))
+    eager = []
+    for it in items:
+        it["si"].zero_all()
+        it["mech"].jacobian(it["si"], save_states=True)
+        eager.append((it["si"].R(), it["si"].vals()))
+    for it, (R, V, sig), (Re, Ve) in zip(items, want, eager):
+        assert rel(Re, R) < RTOL and rel(Ve, V) < RTOL
+    b = g.Batch([it["mech"] for it in items], [it["si"] for it in items])
     blocks, ms = b.tune(reps=2)
     assert (ms > 0).all()
     b.capture()
-    other = g.SolInfo(items[0][0])
+    other = g.SolInfo(items[0]["disc"])
     for rep in range(3):
-        for _, _, si in items:  # scribble over the outputs the replay must overwrite
-            si.set(R=np.full(si.disc.num_dofs, 7.0))
+        for it in items:  # scribble over the outputs, then leave a zero pending
+            it["si"].set(R=np.full(it["disc"].num_dofs, 7.0), vals=np.full(it["om"].cols.size, 5.0))
+            it["si"].zero_all()
         other.zero_all()
-        items[0][1].residual(other, save_states=False)
+        items[0]["mech"].residual(other, save_states=False)
         other.norm_R()
         b.run()
         g.sync()
-        for (_, _, si), (R, V) in zip(items, ref):
-            assert np.array_equal(si.R(), R)
-            assert np.array_equal(si.vals(), V)
+        for it, (R, V, sig), (Re, Ve) in zip(items, want, eager):
+            Rg, Vg = it["si"].R(), it["si"].vals()
+            assert np.array_equal(Rg, Re) and np.array_equal(Vg, Ve), rep
+            assert rel(Rg, R) < RTOL and rel(Vg, V) < RTOL, rep
+            assert rel(it["st"].get("sigma"), sig) < RTOL, rep
+            assert np.array_equal(it["si"].dMdu(), np.zeros(it["disc"].num_dofs))
+
+
+def test_assembly_refuses_more_element_sets_than_materials():
+    """goal_disc_set_elem_sets after goal_mech_create may raise the set count
+    past the material table: assembly, QoI and features refuse to launch."""
+    disc, st, mech, x, om, ost, mats = make(1, 1, 1, n=2)
+    si = g.SolInfo(disc)
+    es = np.zeros(disc.num_elems, dtype=np.int32)
+    es[::2] = 1
+    disc.set_elem_sets(2, es)
+    for call in (lambda: mech.jacobian(si), lambda: mech.residual(si), lambda: mech.qoi(g.QOI_AVG_DISP, si),
+                 lambda: g.learn_features(mech, 0.25)):
+        with pytest.raises(g.GoalError) as err:
+            call()
+        assert err.value.code == g.ERR_INVALID
 
 
 @pytest.mark.parametrize("order,quad,n,tangent", [(2, 2, 4, 64), (2, 2, 4, 32), (1, 1, 7, 64), (1, 2, 5, 64)])

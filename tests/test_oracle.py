@@ -17,7 +17,9 @@ transposed operand) fails it:
   * transposed scatter = transpose of the primal Jacobian;
   * QoIs: closed forms (constant displacement, uniaxial stretch) and FD of dMdu;
   * DWR: partition-of-unity identity of the localized residual, set_error
-    conservation, compute_error / sum_contribs by brute force.
+    conservation, compute_error / sum_contribs by brute force;
+  * set_coarse = the base mesh's P1 interpolant at every nested vertex
+    (barycentric evaluation in the parent tet), affine fields unchanged.
 """
 import numpy as np
 import pytest
@@ -323,6 +325,61 @@ def test_dwr_partition_of_unity():
         sp_ = e[3::4].sum()
         want = zd[3::4] @ Rn[3::4] + zc[3::4] @ (R[3::4] - Rn[3::4])
         assert abs(sp_ - want) < 1e-10 * np.abs(R).sum()
+
+
+def _p1_interpolant_at_fine_vertices(coords, conn, fc, fconn, parent, zb):
+    """The base mesh's P1 interpolant of the nodal field zb [nv, 4], evaluated
+    at every fine vertex from its definition: barycentric coordinates of the
+    vertex in the base tet that holds its child element (parent map), then
+    sum_k lambda_k zb[conn[E][k]].  Independent of edge_parents."""
+    out = np.full((fc.shape[0], 4), np.nan)
+    for c, E in enumerate(parent):
+        X = coords[conn[E]]                      # 4 x 3 base vertices
+        A = np.vstack([np.ones(4), X.T])         # [1; x; y; z] per vertex
+        for v in fconn[c]:
+            lam = np.linalg.solve(A, np.concatenate([[1.0], fc[v]]))
+            assert lam.min() > -1e-12            # the vertex lies in its parent tet
+            out[v] = lam @ zb[conn[E]]
+    return out
+
+
+@pytest.mark.parametrize("seed", [2, 9])
+def test_set_coarse_is_the_base_p1_interpolant(seed):
+    """Nested::set_coarse (src/goal_nested.cpp:369-393) pinned from the
+    definition of the coarse field: on the nested mesh it is the base mesh's
+    P1 interpolant, so every new vertex gets the base field evaluated there.
+    Checked for all four components (u and p) on a jittered base mesh with a
+    random field, with the new vertices' values scrambled before the call.
+    A dropped component, wrong parents or wrong weights fail it."""
+    coords, conn = g.mesh_box(2, 2, 3, order=1, jitter=0.15, seed=seed)
+    fc, fconn, parent, ep = g.mesh_refine(coords, conn)
+    nv = coords.shape[0]
+    rng = np.random.default_rng(seed)
+    zb = rng.standard_normal((nv, 4))
+    z = rng.standard_normal((fc.shape[0], 4)) * 100.0   # garbage on new vertices
+    z[:nv] = zb
+    zf = z.ravel().copy()
+    O.set_coarse(zf, ep, nv)
+    want = _p1_interpolant_at_fine_vertices(coords, conn, fc, fconn, parent, zb)
+    assert not np.isnan(want).any()
+    got = zf.reshape(-1, 4)
+    assert np.array_equal(got[:nv], zb)                 # old vertices untouched
+    assert np.abs(got - want).max() < 1e-13 * np.abs(zb).max()
+
+
+def test_set_coarse_keeps_affine_fields():
+    """An affine field (u and p) is its own P1 interpolant: after set_coarse
+    every nested vertex carries the affine field's value at its position."""
+    coords, conn = g.mesh_box(3, 2, 2, order=1, jitter=0.1, seed=4)
+    fc, fconn, parent, ep = g.mesh_refine(coords, conn)
+    A = np.array([[0.3, -1.2, 0.5], [2.0, 0.1, -0.7], [-0.4, 0.9, 1.1], [1.5, -0.5, 0.25]])
+    b = np.array([0.1, -0.2, 0.3, 4.0])
+    exact = fc @ A.T + b
+    z = exact.copy()
+    z[coords.shape[0]:] = -7.0
+    zf = z.ravel().copy()
+    O.set_coarse(zf, ep, coords.shape[0])
+    assert np.abs(zf.reshape(-1, 4) - exact).max() < 1e-13 * np.abs(exact).max()
 
 
 def test_error_indicator_helpers():
