@@ -88,6 +88,27 @@ void require(bool ok, const char* msg) {
   if (!ok) throw std::invalid_argument(msg);
 }
 
+// ------------------------------------------- allocation-free assembly scratch
+// Element residuals and element matrices of goal_asm_jacobian_noalloc live in
+// this file-scope device array instead of buffers the library allocates.  Each
+// GPU holds its own copy at its own address until the process ends.  In host
+// code `scratch` names a host-side stand-in, not device memory: host code uses
+// only the address hipGetSymbolAddress returned for the current device.
+constexpr int64_t kNoallocScratchDoubles = int64_t(1) << 23;  // 64 MiB
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+double* g_noalloc_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (called from the context set-up).
+void noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) throw HipError(GOAL_ERR_HIP, "device index beyond the scratch table");
+  if (g_noalloc_ptr[device]) return;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_noalloc_ptr[device] = static_cast<double*>(p);
+}
+
 // ---------------------------------------------------------------- context
 struct Context {
   bool ready = false;
@@ -96,7 +117,7 @@ struct Context {
   hipStream_t side = nullptr;        // gathers of a chunked assembly (default priority)
   hipEvent_t chunk_ev[65] = {};      // chunk boundaries of a chunked assembly
   hipEvent_t events[256] = {};
-  double* scratch = nullptr;  // reductions: kReduceBlocks partials + results
+  double* reduce = nullptr;  // reductions: kReduceBlocks partials + results
   std::mutex mu;
 };
 Context g_ctx;
@@ -113,14 +134,15 @@ Context& ctx() {
   HIPCHK(hipStreamCreateWithFlags(&g_ctx.side, hipStreamNonBlocking));
   for (auto& ev : g_ctx.events) HIPCHK(hipEventCreate(&ev));
   for (auto& ev : g_ctx.chunk_ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  HIPCHK(hipMalloc(&g_ctx.scratch, sizeof(double) * (goal::kReduceBlocks + 16)));
+  HIPCHK(hipMalloc(&g_ctx.reduce, sizeof(double) * (goal::kReduceBlocks + 16)));
+  noalloc_setup(g_ctx.device);
   g_ctx.ready = true;
   return g_ctx;
 }
 
 hipStream_t stream() { return ctx().stream; }
-double* scratch() { return ctx().scratch; }
-double* result_slot(int k) { return ctx().scratch + goal::kReduceBlocks + k; }
+double* reduce_scratch() { return ctx().reduce; }
+double* result_slot(int k) { return ctx().reduce + goal::kReduceBlocks + k; }
 
 template <typename T>
 struct DevBuf {
@@ -452,7 +474,7 @@ void run_chunked(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool wa
   g.inc_ptr = d->inc_ptr.p;
   g.posi = d->posi.p;
   g.row_ptr = d->row_ptr.p;
-  g.emr = m->emr.p;
+  g.emr = a.emr;
   g.em = a.em;
   g.em32 = a.em32;
   g.R = s->R.p;
@@ -490,18 +512,28 @@ void run_chunked(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool wa
   if (want_vals) s->zero_vals = false;
 }
 
-void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool want_vals) {
+// Element buffers between the two passes: the mechanics' own (allocated on
+// first use), or, with emr_buf/em_buf given, the caller's.
+// With emr_buf given the assembly also stays in one pass (the chunked path
+// keeps a host-side table it builds on first use); the two give the same bits.
+void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool want_vals, double* emr_buf = nullptr,
+                  double* em_buf = nullptr) {
+  const bool own_buffers = emr_buf == nullptr;
   goal_disc d = m->d;
   const int64_t nd = 4 * (int64_t)d->mesh.nodes_per_elem;
-  if (m->emr.n < d->mesh.num_elems * nd) m->emr.alloc(d->mesh.num_elems * nd);
-  if (want_vals && m->em.n < d->mesh.num_elems * nd * nd) m->em.alloc(d->mesh.num_elems * nd * nd);
+  if (!emr_buf) {
+    if (m->emr.n < d->mesh.num_elems * nd) m->emr.alloc(d->mesh.num_elems * nd);
+    if (want_vals && m->em.n < d->mesh.num_elems * nd * nd) m->em.alloc(d->mesh.num_elems * nd * nd);
+    emr_buf = m->emr.p;
+    em_buf = m->em.p;
+  }
   a.rec = d->rec.p;
-  a.emr = m->emr.p;
+  a.emr = emr_buf;
   const bool f32 = want_vals && m->tangent == 32;
-  a.em = (want_vals && !f32) ? m->em.p : nullptr;
-  a.em32 = f32 ? reinterpret_cast<float*>(m->em.p) : nullptr;
+  a.em = (want_vals && !f32) ? em_buf : nullptr;
+  a.em32 = f32 ? reinterpret_cast<float*>(em_buf) : nullptr;
   if (m->timing) HIPCHK(hipEventRecord(m->ev[0], stream()));
-  if (chunked(m, want_vals)) {
+  if (own_buffers && chunked(m, want_vals)) {
     run_chunked(m, s, kind, a, want_vals);
     return;
   }
@@ -522,7 +554,7 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
   g.inc_ptr = d->inc_ptr.p;
   g.posi = d->posi.p;
   g.row_ptr = d->row_ptr.p;
-  g.emr = m->emr.p;
+  g.emr = a.emr;
   g.em = a.em;
   g.em32 = a.em32;
   g.R = s->R.p;
@@ -1370,7 +1402,7 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
     double* z = s->z.p;
     HIPCHK(goal::launch_block_inverse(nn, rp, cols, A, s->binv.p, s->solve_status.p, stream()));
     auto norm2 = [&](const double* v) {  // waits
-      HIPCHK(goal::launch_reduce(1, v, nullptr, n, scratch(), result_slot(2), stream()));
+      HIPCHK(goal::launch_reduce(1, v, nullptr, n, reduce_scratch(), result_slot(2), stream()));
       double r = 0.0;
       HIPCHK(hipMemcpyAsync(&r, result_slot(2), sizeof(double), hipMemcpyDeviceToHost, stream()));
       sync();
@@ -1467,7 +1499,7 @@ int goal_solinfo_norm_R(goal_solinfo s, double* norm) {
   return guard([&]() -> int {
     require(s && norm, "bad arguments");
     materialize(s);
-    HIPCHK(goal::launch_reduce(1, s->R.p, nullptr, s->R.n, scratch(), result_slot(0), stream()));
+    HIPCHK(goal::launch_reduce(1, s->R.p, nullptr, s->R.n, reduce_scratch(), result_slot(0), stream()));
     double v = 0.0;
     HIPCHK(hipMemcpyAsync(&v, result_slot(0), sizeof(double), hipMemcpyDeviceToHost, stream()));
     sync();
@@ -1480,7 +1512,7 @@ int goal_solinfo_dot_R(goal_solinfo s, const double* z_dev, double* dot) {
   return guard([&]() -> int {
     require(s && z_dev && dot, "bad arguments");
     materialize(s);
-    HIPCHK(goal::launch_reduce(2, s->R.p, z_dev, s->R.n, scratch(), result_slot(0), stream()));
+    HIPCHK(goal::launch_reduce(2, s->R.p, z_dev, s->R.n, reduce_scratch(), result_slot(0), stream()));
     double v = 0.0;
     HIPCHK(hipMemcpyAsync(&v, result_slot(0), sizeof(double), hipMemcpyDeviceToHost, stream()));
     sync();
@@ -1605,6 +1637,29 @@ int goal_asm_jacobian(goal_mech m, goal_solinfo s, int adjoint, int save_states)
     goal::AsmArgs a = asm_args(m);
     a.save_state = save_states ? 1 : 0;
     run_assembly(m, s, adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN, a, true);
+    return finish_assembly(m);
+  });
+}
+
+int goal_asm_jacobian_noalloc(goal_mech m, goal_solinfo s, int adjoint, int save_states) {
+  return guard([&]() -> int {
+    require(m && s && s->d == m->d, "goal_asm_jacobian_noalloc: bad arguments");
+    goal::AsmArgs a = asm_args(m);  // ctx() is up: the mechanics was created through it
+    const int64_t ne = m->d->mesh.num_elems, nd = 4 * (int64_t)m->d->mesh.nodes_per_elem;
+    // element matrices (FP64; FP32 ones take half) first, then element residuals
+    const int64_t need = ne * nd * nd + ne * nd;
+    if (need > kNoallocScratchDoubles)
+      return set_error(GOAL_ERR_INVALID,
+                       "goal_asm_jacobian_noalloc: %lld elements need %lld doubles of scratch, more than its "
+                       "%lld (at most %lld %s tets)",
+                       (long long)ne, (long long)need, (long long)kNoallocScratchDoubles,
+                       (long long)(kNoallocScratchDoubles / (nd * nd + nd)), nd == 40 ? "ten-node" : "four-node");
+    double* base = g_noalloc_ptr[ctx().device];
+    require(base != nullptr, "goal_asm_jacobian_noalloc: scratch address not set up for this device");
+    a.save_state = save_states ? 1 : 0;
+    // every value the gather reads is written by this call's element pass, so
+    // nothing left from an earlier call needs clearing
+    run_assembly(m, s, adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN, a, true, base + ne * nd * nd, base);
     return finish_assembly(m);
   });
 }
@@ -1785,7 +1840,7 @@ int goal_asm_qoi(goal_mech m, goal_solinfo s, int kind, int elem_set, double* va
     q.dMdu = s ? s->dMdu.p : nullptr;
     q.status = m->status.p;
     HIPCHK(goal::launch_qoi(d->mesh.order, m->quad_order, q, stream()));
-    HIPCHK(goal::launch_reduce(0, m->elem_value.p, nullptr, d->mesh.num_elems, scratch(), result_slot(1), stream()));
+    HIPCHK(goal::launch_reduce(0, m->elem_value.p, nullptr, d->mesh.num_elems, reduce_scratch(), result_slot(1), stream()));
     HIPCHK(hipMemcpyAsync(value, result_slot(1), sizeof(double), hipMemcpyDeviceToHost, stream()));
     sync();
     if (elem_values) m->elem_value.download(elem_values, d->mesh.num_elems);
@@ -2303,7 +2358,7 @@ int goal_dwr_compute_error(goal_disc d, const double* e, double* eta) {
 int goal_dwr_sum_contribs(goal_disc d, const double* e, double* bound) {
   return guard([&]() -> int {
     require(d && e && bound, "bad arguments");
-    HIPCHK(goal::launch_sum_contribs(d->vertex_nodes.p, d->vertex_nodes.n, e, scratch(), result_slot(2), stream()));
+    HIPCHK(goal::launch_sum_contribs(d->vertex_nodes.p, d->vertex_nodes.n, e, reduce_scratch(), result_slot(2), stream()));
     HIPCHK(hipMemcpyAsync(bound, result_slot(2), sizeof(double), hipMemcpyDeviceToHost, stream()));
     sync();
     return GOAL_OK;

@@ -269,6 +269,17 @@ int goal_asm_residual(goal_mech m, goal_solinfo s, int save_states);
  * src/goal_pressure.cpp:170-185); adjoint != 0 scatters the transpose
  * (scatter_adjoint, src/goal_displacement.cpp:196-214, src/goal_pressure.cpp:187-203). */
 int goal_asm_jacobian(goal_mech m, goal_solinfo s, int adjoint, int save_states);
+/* goal_asm_jacobian without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The element
+ * residuals and matrices between the two passes live in a 64 MiB array that the
+ * library holds on each GPU until the process ends, so a mesh may have at most
+ * 5,115 ten-node or 30,840 four-node tets (larger ones: GOAL_ERR_INVALID, and
+ * nothing is launched).  Results are bitwise those of goal_asm_jacobian.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (other
+ * streams or threads) race.  Under capture, set the mechanics' "check" option
+ * to 0 (the status read-back synchronizes). */
+int goal_asm_jacobian_noalloc(goal_mech m, goal_solinfo s, int adjoint, int save_states);
 /* Functional::compute / QoI<ST>, QoI<FADT> (src/goal_functional.cpp:62-70,
  * src/goal_qoi.cpp:20-81, src/goal_avg_disp.cpp:16-21, src/goal_avg_vm.cpp:32-61).
  * kind GOAL_QOI_*; elem_set: AvgVM element set (-1 = all); value: J(u);
