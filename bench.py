@@ -753,10 +753,8 @@ def main() -> int:
     gather_ms = statistics.mean(gg for _, gg in splits)
     log(f"element pass {element_ms:.3f} ms, gather {gather_ms:.3f} ms")
 
-    def chunks(tangent):   # the library's choice for the "overlap" option
-        if args.overlap >= 0:
-            return args.overlap
-        return 32 if tangent == 32 and wl["model"] == 1 else 0
+    def chunks(tangent):   # the library's choice for the "overlap" option (automatic: one pass)
+        return max(args.overlap, 0)
 
     # the other element-matrix precision, same mesh and iterate, for comparison
     other = 32 if args.tangent == 64 else 64

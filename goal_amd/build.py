@@ -21,10 +21,16 @@ ARCH = os.environ.get("GOAL_AMD_ARCH", "gfx950")
 
 # Minimum waves per SIMD requested for the element kernel (register cap).
 ELEM_WAVES = int(os.environ.get("GOAL_ELEM_WAVES", "3"))
+# Elements per workgroup of the matrix-core (neo-Hookean ten-node, 4-point) element kernel.
+MF_G = int(os.environ.get("GOAL_MF_G", "6"))
+# 0 builds the vector-ALU element-matrix phase for every kernel instead.
+MF = int(os.environ.get("GOAL_MF", "1"))
 
 COMMON = [
     "-O3",
     f"-DGOAL_ELEM_WAVES={ELEM_WAVES}",
+    f"-DGOAL_MF_G={MF_G}",
+    f"-DGOAL_MF={MF}",
     "-fPIC",
     "-std=c++17",
     "-Wall",

@@ -70,16 +70,16 @@ struct ModelOf<MODEL_J2> {
 };
 
 // What phase C1 leaves in LDS for the tangent columns (phase C2, or phase E
-// for neo-Hookean).
-template <int MODEL>
+// for neo-Hookean, kept in the element-matrix type S).
+template <int MODEL, typename S = double>
 struct QPStore {
   dev::QP qp;
   dev::Stab st;
   typename ModelOf<MODEL>::type m;
 };
-template <>
-struct QPStore<MODEL_NEOHOOKEAN> {
-  dev::NHBase b;
+template <typename S>
+struct QPStore<MODEL_NEOHOOKEAN, S> {
+  dev::NHBaseT<S> b;
 };
 
 // Workgroup shape.  Phase E gives one lane to each column of an element
@@ -87,11 +87,24 @@ struct QPStore<MODEL_NEOHOOKEAN> {
 // wavefronts: 6 ten-node tets on 256 lanes, 16 four-node tets on 256.  Four
 // waves per workgroup let three workgroups share a CU at W = 3 waves per SIMD
 // (the register budget); five-wave workgroups were only ever placed one per CU.
+//
+// Neo-Hookean ten-node tets with four points (MF) form their element matrices
+// on the matrix cores instead, in tiles of 16 columns per wavefront: G = 6
+// elements on four waves, their 18 tiles dealt round-robin (measured on the
+// P2 64^3 cube: FP64 element pass 7.04 ms at G = 6 against 7.69 ms at G = 4,
+// where every wave takes one element).
+#ifndef GOAL_MF_G
+#define GOAL_MF_G 6
+#endif
+#ifndef GOAL_MF
+#define GOAL_MF 1  // 0: the vector-ALU phase E for every kernel
+#endif
 template <int NN, int NQ, int MODEL>
 struct Layout {
   static constexpr int ND = 4 * NN;
-  static constexpr int G = (NN == 10) ? 6 : (NQ == 1 ? 16 : 4);
-  static constexpr int T = ((G * ND + 63) / 64) * 64;
+  static constexpr bool MF = GOAL_MF && MODEL == MODEL_NEOHOOKEAN && NN == 10 && NQ == 4;
+  static constexpr int G = MF ? GOAL_MF_G : (NN == 10) ? 6 : (NQ == 1 ? 16 : 4);
+  static constexpr int T = MF ? 256 : ((G * ND + 63) / 64) * 64;
   static constexpr int W = (MODEL == MODEL_J2) ? 1 : GOAL_ELEM_WAVES;
 };
 
@@ -105,15 +118,15 @@ struct SBase {
   int32_t conn[G][NN];
   int32_t rec[G][NN];
 };
-template <int NQ, int MODEL, int G>
+template <int NQ, int MODEL, int G, typename S>
 struct SJac {
   double D[G][NQ][13][13];
   QPStore<MODEL> qs[G][NQ];
 };
 // neo-Hookean: phase E contracts the closed-form tangent itself
-template <int NQ, int G>
-struct SJac<NQ, MODEL_NEOHOOKEAN, G> {
-  QPStore<MODEL_NEOHOOKEAN> qs[G][NQ];
+template <int NQ, int G, typename S>
+struct SJac<NQ, MODEL_NEOHOOKEAN, G, S> {
+  QPStore<MODEL_NEOHOOKEAN, S> qs[G][NQ];
 };
 template <int NN, int NQ, int G>
 struct SLoc {
@@ -135,16 +148,17 @@ struct SPrefetch {
   double px[G][4 * NN];
 };
 struct SNoPrefetch {};
-// (FP64 element matrices and residuals: the FP32-tangent kernels run chunked
-// beside the gather, where it measured slightly slower, 10.50 -> 10.55 ms)
+// (FP64 element matrices and residuals, and the matrix-core kernels of either
+// precision; the vector-ALU FP32-tangent kernels measured slightly slower
+// with it, 10.50 -> 10.55 ms, chunked beside the gather)
 template <int NN, int NQ, int MODEL, int KIND, typename TE>
 constexpr bool kPrefetch = MODEL == MODEL_NEOHOOKEAN && KIND != ASM_LOCALIZE && Layout<NN, NQ, MODEL>::T >= 128 &&
-                           std::is_same<TE, double>::value;
+                           (std::is_same<TE, double>::value || Layout<NN, NQ, MODEL>::MF);
 
 template <int NN, int NQ, int MODEL, int KIND, int G, typename TE = double>
 struct Smem : std::conditional<kPrefetch<NN, NQ, MODEL, KIND, TE>, SPrefetch<NN, G>, SNoPrefetch>::type {
   using Ext = typename std::conditional<
-      KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT, SJac<NQ, MODEL, G>,
+      KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT, SJac<NQ, MODEL, G, TE>,
       typename std::conditional<KIND == ASM_LOCALIZE, SLoc<NN, NQ, G>, SNone>::type>::type;
   SBase<NN, NQ, G> b;
   Ext ext;
@@ -234,7 +248,7 @@ __device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[
 // with node b's shape function (dev::nh_direction), then the rows as above.
 template <int NN, int NQ, bool TR, typename T>
 __device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* wdv,
-                                      const QPStore<MODEL_NEOHOOKEAN>* qs, int col, const int32_t* rec,
+                                      const QPStore<MODEL_NEOHOOKEAN, T>* qs, int col, const int32_t* rec,
                                       T* em) {
   const int b = col >> 2, kc = col & 3;
   T acc[NN][4];
@@ -258,6 +272,121 @@ __device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* w
     }
   }
   store_column<NN, TR, T, std::is_same<T, float>::value>(acc, col, rec, em);
+}
+
+// Matrix-core form of the neo-Hookean element matrix (four points).  With the
+// point index q as the inner dimension k of a 16x16x4 product,
+//   K[(a,i), col] = sum_J sum_q G_q[a][J] C_q[3i+J][col]                      (i < 3)
+//   K[(a,3), col] = sum_q N_q[a] C_q[9][col] + sum_J sum_q G_q[a][J] C_q[10+J][col]
+// where G_q, N_q are the shape-function gradients and values at point q (rows
+// a = element nodes, zero past NN) and C_q[:, col] = D_q B_q[:, col] is the
+// point tangent applied to column col (dev::nh_direction, w dv folded in).
+// A wavefront takes 16 columns at a time: lane l = 16 q + n evaluates C_q for
+// column 16 t + n, which is exactly the B operand of the product (k = q, n),
+// and holds G_q[n][J], N_q[n] as the A operands (m = n, k = q).  Thirteen
+// products give the four 16x16 row blocks (i = 0..3) of the tile; lane l then
+// holds four rows of column 16 t + (l % 16): a = 4 (l / 16) + r for the FP32
+// product and a = 4 r + (l / 16) for the FP64 one, r = 0..3 (measured on
+// gfx950 with a probe of known products).
+// T: FP64 (v_mfma_f64_16x16x4f64) or FP32 (v_mfma_f32_16x16x4f32) element
+// matrices, accumulated in their own precision.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc = f64x4;
+  __device__ static acc mac(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int g, int r) { return 4 * r + g; }
+};
+template <>
+struct Mfma<float> {
+  using acc = f32x4;
+  __device__ static acc mac(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int g, int r) { return 4 * g + r; }
+};
+
+// Tiles of the element's columns: displacement columns of five nodes (15
+// lanes) per tile, then the pressure columns (up to 16 nodes per tile), so
+// every lane of a tile takes the same branch of dev::nh_direction.
+template <int NN>
+struct MfTiles {
+  static constexpr int DT = (3 * NN + 14) / 15;
+  static constexpr int PT = (NN + 15) / 16;
+  static constexpr int NT = DT + PT;
+  // element-matrix column of lane n in tile t, or -1
+  __device__ static int col(int t, int n) {
+    if (t < DT) {
+      const int b = 5 * t + n / 3;
+      return (n < 15 && b < NN) ? 4 * b + n % 3 : -1;
+    }
+    const int b = 16 * (t - DT) + n;
+    return b < NN ? 4 * b + 3 : -1;
+  }
+};
+
+template <int NN, bool TR, typename T>
+__device__ inline void emit_tile_mfma(const double (*sh)[NN][4], const double* wdv,
+                                      const QPStore<MODEL_NEOHOOKEAN, T>* qs, const int32_t* rec, T* em, int lane,
+                                      int t) {
+  constexpr int ND = 4 * NN;
+  using M = Mfma<T>;
+  using acc = typename M::acc;
+  const int q = lane >> 4, n = lane & 15, g = lane >> 4;
+  T aN = (T)0, aG0 = (T)0, aG1 = (T)0, aG2 = (T)0;
+  if (n < NN) {
+    const double* sa = sh[q][n];
+    aN = (T)sa[0];
+    aG0 = (T)sa[1];
+    aG1 = (T)sa[2];
+    aG2 = (T)sa[3];
+  }
+  const int col = MfTiles<NN>::col(t, n);
+  T C[13];
+  if (col >= 0) {
+    const double w = wdv[q];
+    const double* shb = sh[q][col >> 2];
+    const T G[3] = {(T)(w * shb[1]), (T)(w * shb[2]), (T)(w * shb[3])};
+    dev::nh_direction<T>(qs[q].b, col & 3, (T)(w * shb[0]), G, C);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 13; ++r) C[r] = (T)0;
+  }
+  acc k0 = {0, 0, 0, 0}, k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
+  k0 = M::mac(aG0, C[0], k0);
+  k1 = M::mac(aG0, C[3], k1);
+  k2 = M::mac(aG0, C[6], k2);
+  k3 = M::mac(aN, C[9], k3);
+  k0 = M::mac(aG1, C[1], k0);
+  k1 = M::mac(aG1, C[4], k1);
+  k2 = M::mac(aG1, C[7], k2);
+  k3 = M::mac(aG0, C[10], k3);
+  k0 = M::mac(aG2, C[2], k0);
+  k1 = M::mac(aG2, C[5], k1);
+  k2 = M::mac(aG2, C[8], k2);
+  k3 = M::mac(aG1, C[11], k3);
+  k3 = M::mac(aG2, C[12], k3);
+  if (col < 0) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int a = M::row(g, r);
+    if (a < NN) {
+      if constexpr (!TR) {
+        T* o = em + (int64_t)rec[a] * (4 * ND) + col;
+        o[0] = k0[r];
+        o[ND] = k1[r];
+        o[2 * ND] = k2[r];
+        o[3 * ND] = k3[r];
+      } else {
+        T* o = em + (int64_t)rec[col >> 2] * (4 * ND) + (col & 3) * ND + 4 * a;
+        o[0] = k0[r];
+        o[1] = k1[r];
+        o[2] = k2[r];
+        o[3] = k3[r];
+      }
+    }
+  }
 }
 
 template <int NN, int NQ, int MODEL, int KIND, typename TE>
@@ -322,15 +451,17 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     __syncthreads();
 
     // ---- B: shape functions at the quadrature points (u/uw/p/pw at_point)
-    if constexpr (MODEL == MODEL_NEOHOOKEAN && std::is_same<TE, double>::value) {
+    if constexpr (MODEL == MODEL_NEOHOOKEAN && (std::is_same<TE, double>::value || Lay::MF)) {
       // grad lambda and det once per element, staged in the stress slots of
       // the element (phase C1 writes them after the barrier below); each
       // (element, point, node) lane then reads the rows it needs by address,
       // not through per-lane selects over register arrays.  Same arithmetic
       // as dev::shape, so the same bits.  Measured on the P2 64^3 cube: FP64
-      // element pass 8.55 -> 8.19 ms.  The FP32-tangent kernels keep the
-      // register form: their chunked assembly shares the CUs with the
-      // gather's LDS atomics and went from 10.51 to 10.93 ms with this one.
+      // element pass 8.55 -> 8.19 ms.  The vector-ALU FP32-tangent kernels
+      // keep the register form: their chunked assembly shares the CUs with
+      // the gather's LDS atomics and went from 10.51 to 10.93 ms with this
+      // one.  The matrix-core FP32 kernels use it (element pass 6.60 ->
+      // 6.05 ms with the prefetch below).
       for (int el = tid; el < nv; el += T) {
         double gl[4][3];
         double* o = &s.sig[el][0][0];
@@ -565,7 +696,14 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
     TE* em;
     if constexpr (std::is_same<TE, float>::value) em = a.em32;
     else em = a.em;
-    if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
+    if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN && Lay::MF) {
+      // matrix cores: the (element, tile) tasks dealt round-robin over the waves
+      constexpr int NTL = MfTiles<NN>::NT;
+      for (int task = tid >> 6; task < nv * NTL; task += T / 64) {
+        const int el = task / NTL;
+        emit_tile_mfma<NN, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], s.rec[el], em, tid & 63, task % NTL);
+      }
+    } else if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
       if constexpr (T == 256 && G * NN <= 64 && std::is_same<TE, double>::value) {
         // displacement columns on waves 0-2, pressure columns on wave 3: the
         // closed-form column takes a different (shorter) path for pressure,
@@ -941,11 +1079,21 @@ hipError_t launch_gather(int order, const GatherArgs& g, int blocks, hipStream_t
   return hipGetLastError();
 }
 
+namespace {
+template <int NN, int NQ>
+int group_size_of(int model) {
+  switch (model) {
+    case MODEL_ELASTIC: return Layout<NN, NQ, MODEL_ELASTIC>::G;
+    case MODEL_NEOHOOKEAN: return Layout<NN, NQ, MODEL_NEOHOOKEAN>::G;
+    default: return Layout<NN, NQ, MODEL_J2>::G;
+  }
+}
+}  // namespace
+
 int element_group_size(int order, int quad_order, int model) {
   const int nq = quad_order == 1 ? 1 : 4;
-  if (order == 2) return nq == 1 ? Layout<10, 1, MODEL_NEOHOOKEAN>::G : Layout<10, 4, MODEL_NEOHOOKEAN>::G;
-  (void)model;  // G depends on the element and point counts only
-  return nq == 1 ? Layout<4, 1, MODEL_NEOHOOKEAN>::G : Layout<4, 4, MODEL_NEOHOOKEAN>::G;
+  if (order == 2) return nq == 1 ? group_size_of<10, 1>(model) : group_size_of<10, 4>(model);
+  return nq == 1 ? group_size_of<4, 1>(model) : group_size_of<4, 4>(model);
 }
 
 namespace {

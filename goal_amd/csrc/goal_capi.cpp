@@ -246,7 +246,8 @@ struct goal_mech_s {
   int blocks = 0, xcd_remap = 1, check = 1;
   int tangent = 64;  // 32: FP32 element matrices for the Jacobian kinds
   // neo-Hookean assemblies in chunks, gathers on a second stream: > 1 chunks,
-  // 0 off, -1 automatic (32 chunks for FP32-tangent Jacobians, where it pays)
+  // 0 off, -1 automatic (one pass: with matrix-core element matrices chunking
+  // no longer pays)
   int overlap = -1;
   int overlap_blocks = 0;  // element workgroups per chunk (0: 5/6 of the resident grid)
   DevBuf<goal::MatParams> mats;
@@ -423,13 +424,16 @@ void materialize(goal_solinfo s) {
 // gathered on the side stream, while chunk c + 1 computes.  The element pass
 // is issue-bound and the gather HBM-bound, so the two share the GPU; the
 // element grid is held below the resident size to leave the gather room.
-// Measured on the P2 64^3 cube: FP32-tangent Jacobian 11.97 -> 10.64 ms at 32
-// chunks, FP64 unchanged (its element pass already saturates HBM with the
-// gather's share), hence the automatic setting.
+// Measured on the P2 64^3 cube with the vector-ALU element matrices: FP32-
+// tangent Jacobian 11.97 -> 10.64 ms at 32 chunks, FP64 unchanged (its
+// element pass already saturates HBM with the gather's share).  With the
+// matrix-core element matrices one pass is faster for both (FP32 tangent
+// 9.56 ms in one pass, 9.84 ms in 32 chunks), so the automatic setting is
+// one pass.
 // Same kernels and the same arithmetic as one pass: results are bitwise equal.
 int chunks_of(goal_mech m, bool want_vals) {
-  if (m->overlap >= 0) return m->overlap;
-  return (want_vals && m->tangent == 32) ? 32 : 0;
+  (void)want_vals;
+  return m->overlap >= 0 ? m->overlap : 0;
 }
 
 bool chunked(goal_mech m, bool want_vals) {

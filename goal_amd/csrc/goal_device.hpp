@@ -466,12 +466,17 @@ struct Elastic {
 // The reference differentiates the same formulas before simplification
 // (the bulk term 1/2 kappa (J - 1/J) is added by the model and removed again
 // by Mixed), so the two agree to rounding.
-struct NHBase {
-  double F[9], Fi[9], P[9], Q[9], Cinv[9], v[3], h[3];
-  double J, c, tau, beta, ikappa;  // beta = -1/2 (1 + 1/J^2) J
+// S: the type the point's tangent base is kept in (double; float for FP32
+// element matrices, whose columns are evaluated in float anyway).
+template <typename S>
+struct NHBaseT {
+  S F[9], Fi[9], P[9], Q[9], Cinv[9], v[3], h[3];
+  S J, c, tau, beta, ikappa;  // beta = -1/2 (1 + 1/J^2) J
 };
+using NHBase = NHBaseT<double>;
 
-__device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st, NHBase& b) {
+template <typename S>
+__device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st, NHBaseT<S>& b) {
   double trb = 0.0;
   for (int k = 0; k < 9; ++k) trb += q.F[k] * q.F[k];
   const double c = m.mu * q.J * m.Jm53;
@@ -507,8 +512,8 @@ __device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st,
 // These are the columns of the point tangent summed against G; scaling G and
 // N_b by w dv scales the result.  T = float evaluates the FP32 tangent from the
 // FP64 point state.
-template <typename T>
-__device__ inline void nh_direction(const NHBase& b, int k, T N, const T* G, T* C) {
+template <typename T, typename S>
+__device__ inline void nh_direction(const NHBaseT<S>& b, int k, T N, const T* G, T* C) {
   const T c = (T)b.c, J = (T)b.J, tJ = (T)(b.tau * b.J);
   if (k < 3) {
     T FG[3], PG[3], CG[3];
