@@ -128,65 +128,77 @@ def smooth_state(coords, amp=0.02):
     return x
 
 
-def cpu_baseline(n_sample: int, threads: int, wl, reps: int = 5):
-    """Oracle (C++ restatement, dual numbers, column-search scatter) on a sample
-    of the workload: n_sample^3 hexahedra of the cube, or the first 2 n_sample
-    hexahedron layers of the bar's full 48 x 48 cross-section (same cell size)."""
+def host_cores():
+    """Cores this process may run on, and what the host reports: the CPU
+    baseline uses the cores allotted to the process (its affinity set, capped
+    by OMP_NUM_THREADS where the host sets it)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    use = min(avail, int(omp)) if omp.isdigit() and int(omp) > 0 else avail
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return use, {"nproc": os.cpu_count(), "affinity": avail, "omp_num_threads": omp or None, "cpu_model": model}
+
+
+def sample_of(disc, wl, layers):
+    """The first `layers` hexahedron layers (lowest z) of the workload's own
+    mesh as a mesh of their own, with the workload's iterate on its nodes.
+    Elements are numbered layer by layer (6 tets per hexahedron, x fastest),
+    so the sample is an element prefix; its nodes keep their relative order
+    (vertices, then edge nodes)."""
     import numpy as np
 
-    import goal_amd as g
+    nx, ny, _ = wl["dims"]
+    coords, conn, _ = disc.mesh()
+    ne = min(disc.num_elems, layers * 6 * nx * ny)
+    used = np.unique(conn[:ne])
+    remap = np.full(disc.num_nodes, -1, dtype=np.int32)
+    remap[used] = np.arange(used.size, dtype=np.int32)
+    x = disc.soln().reshape(-1, 4)[used].ravel()
+    return coords[used], remap[conn[:ne]], x
+
+
+def cpu_baseline(sample, threads: int, wl, reps: int = 5):
+    """The oracle (C++ restatement: dual numbers, column-search CSR scatter)
+    assembling `sample` (coords, conn, x) with one part per core, each part
+    holding only its own and ghost rows, then the export of ghost rows into
+    their owners -- zero_all + assemble + gather_all of Primal::compute_jacob.
+    Median of `reps` after one warm-up; the phase split is of the median run."""
+    import numpy as np
+
     import oracle as O
 
+    coords, conn, x = sample
     order, quad = wl["order"], wl["quad"]
-    if wl["name"] == "cube":
-        coords, conn = g.mesh_box(n_sample, n_sample, n_sample, order=order, jitter=0.1, seed=0x5EED)
-        x = smooth_state(coords)
-    else:
-        k = max(1, 2 * n_sample)
-        coords, conn = g.mesh_box(k, 48, 48, k / 48.0, 1.0, 1.0, order=order, jitter=0.1, seed=0x5EED)
-        x = bar_state(coords)
     m = O.Mesh(order, coords, conn)
+    parts = O.Parts(m, threads)
     st = O.States(m, wl["model"], quad)
     mt = wl["mat"]
     mats = [[mt["E"], mt["nu"], mt["K"], mt["Y"], mt["c0"]]]
     R = np.zeros(m.num_dofs)
     V = np.zeros(m.nnz)
-    times = []
+    runs = []
     for k in range(reps + 1):
         R[:] = 0.0
         V[:] = 0.0
         t0 = time.perf_counter()
-        O.assemble(O.K_JACOBIAN, m, wl["model"], mats, x, st, quad_order=quad, R=R, vals=V, threads=threads)
+        _, _, sec = parts.assemble(O.K_JACOBIAN, wl["model"], mats, x, st, quad_order=quad, R=R, vals=V)
         dt = time.perf_counter() - t0
         if k > 0:
-            times.append(dt)
-    med = statistics.median(times)
-    return m.num_elems / med, m.num_elems, med, (coords, conn, x, R, V)
-
-
-def accuracy(g, wl, sample):
-    """The GPU assembly of the CPU-baseline sample against the oracle's R and
-    dR/du of the same run (normwise relative), FP64 and FP32-tangent
-    element matrices."""
-    import numpy as np
-
-    coords, conn, x, R, V = sample
-    disc = g.Disc.from_arrays(wl["order"], coords, conn)
-    st = g.States(disc, wl["model"], wl["quad"])
-    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=wl["model"], **wl["mat"])], stabilization=True,
-                       quad_order=wl["quad"])
-    disc.set_soln(x)
-    si = g.SolInfo(disc)
-    out = {}
-    for tangent in (64, 32):
-        mech.set_option("tangent", tangent)
-        si.zero_all()
-        mech.jacobian(si, adjoint=False, save_states=False)
-        r, v = si.R(), si.vals()
-        tag = "" if tangent == 64 else "_fp32_tangent"
-        out["R_rel" + tag] = float(np.linalg.norm(r - R) / np.linalg.norm(R))
-        out["dRdu_rel" + tag] = float(np.linalg.norm(v - V) / np.linalg.norm(V))
-    return out
+            runs.append((dt, sec))
+    runs.sort(key=lambda r: r[0])
+    med, sec = runs[len(runs) // 2]
+    held, total = parts.rows()
+    split = {"zero_s": round(float(sec[0]), 4), "elements_s": round(float(sec[1]), 4),
+             "export_s": round(float(sec[2]), 4), "rows_held_over_rows": round(held / total, 3)}
+    return m.num_elems / med, m.num_elems, med, split, (coords, conn, x, R, V)
 
 
 def _read_csv(path):
@@ -649,8 +661,9 @@ def main() -> int:
                          "(-1: the library's automatic choice, 0: off)")
     ap.add_argument("--overlap-blocks", type=int, default=0, help="element workgroups per chunk (0: 5/6 resident)")
     ap.add_argument("--xcd-remap", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=32, help="hexahedra per side of the CPU baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-layers", type=int, default=8,
+                    help="hexahedron layers (lowest z) of the workload's mesh in the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline parts (0: the cores this process has)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tangent", type=int, default=64, choices=(64, 32),
                     help="element-matrix precision of the measured path (the residual is FP64 either way)")
@@ -813,9 +826,14 @@ def main() -> int:
     cpu = None
     acc = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle on {args.cpu_sample}^3-hex sample, {threads} threads")
-        v, ne_s, med, sample = cpu_baseline(args.cpu_sample, threads, wl)
+        threads, host = host_cores()
+        if args.cpu_threads:
+            threads = args.cpu_threads
+        layers = args.cpu_layers
+        log(f"cpu baseline: oracle on the first {layers} hex layers of this mesh at this iterate, {threads} parts "
+            f"({host})")
+        sample = sample_of(disc, wl, layers)
+        v, ne_s, med, split, sample = cpu_baseline(sample, threads, wl)
         acc = accuracy(g, wl, sample)
         acc["bar_fp64"] = 1e-11
         acc["sample"] = "the cpu_baseline sample, GPU against the oracle's output of the same run"
@@ -826,14 +844,17 @@ def main() -> int:
             "unit": "elem/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"{ne_s} P{order} tets ("
-                      + (f"{args.cpu_sample}^3 hex cube" if wl["name"] == "cube" else
-                         f"first {2 * args.cpu_sample} of the bar's 384 hex layers") +
-                      f", same jitter/material, smooth synthetic iterate), residual+Jacobian with "
-                      f"FAD-{16 if order == 1 else 40} and column-search CSR scatter, median of 5 after 1 warm-up "
-                      f"({med:.2f} s each), host {platform.processor() or platform.machine()}",
+            "sample": f"{ne_s} P{order} tets: the first {layers} of the {wl['dims'][2]} hex layers of this "
+                      f"mesh at this iterate ({iterate['kind']}); zero_all + residual/Jacobian with "
+                      f"FAD-{16 if order == 1 else 40} and column-search CSR scatter + export of ghost rows, "
+                      f"one part per core holding its own and ghost rows; median of 5 after 1 warm-up "
+                      f"({med:.3f} s each); host {host['cpu_model']}, nproc {host['nproc']}, "
+                      f"{host['affinity']} cores in this process's affinity set, OMP_NUM_THREADS "
+                      f"{host['omp_num_threads']}",
+            "host": host,
+            "split": split,
         }
-        log(f"cpu baseline: {v:.0f} elem/s ({elems_per_s / v:.0f}x)")
+        log(f"cpu baseline: {v:.0f} elem/s ({elems_per_s / v:.0f}x), split {split}")
 
     out = {
         "metric": wl["metric"],

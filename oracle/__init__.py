@@ -49,6 +49,11 @@ _sig("oracle_qoi", _i32, _i32, _i32, _i32, _i32, _i64, _i64, _p, _p, _p, _p, _p,
 _sig("oracle_dbc", None, _i64, _p, _p, _i32, _p, _p, _p, _p, _p, _p)
 _sig("oracle_traction", None, _i32, _i64, _p, _p, _p, _p, _p)
 _sig("oracle_set_coarse", None, _p, _p, _i64, _i64)
+_sig("oracle_parts_create", _p, _i32, _i64, _i64, _p, _p, _i32)
+_sig("oracle_parts_destroy", None, _p)
+_sig("oracle_parts_rows", None, _p, _p)
+_sig("oracle_assemble_parts", _i32, _p, _i32, _i32, _i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+     _p, _p, _p, _p)
 _sig("oracle_compute_error", None, _i32, _i64, _p, _p, _p)
 _sig("oracle_sum_contribs", C.c_double, _i64, _p, _p)
 _sig("oracle_set_error", None, _i64, _p, _p, _i64, _p)
@@ -227,3 +232,47 @@ def set_error(parent, child, num_base):
     base = np.zeros(num_base)
     lib.oracle_set_error(parent.size, _ptr(parent), _ptr(_f64(child)), num_base, _ptr(base))
     return base
+
+
+class Parts:
+    """The rank-per-part assembly plan of a mesh (one part per thread: a
+    contiguous element range and only the rows its elements touch)."""
+
+    def __init__(self, mesh: Mesh, threads: int):
+        self.mesh = mesh
+        self.threads = threads
+        self.h = lib.oracle_parts_create(mesh.order, mesh.num_nodes, mesh.num_elems, _ptr(mesh.conn),
+                                         _ptr(mesh.row_ptr), threads)
+
+    def rows(self):
+        """(rows held over all parts, rows of the mesh)."""
+        out = np.zeros(2, dtype=np.int64)
+        lib.oracle_parts_rows(self.h, _ptr(out))
+        return int(out[0]), int(out[1])
+
+    def assemble(self, kind, model, mats, x, states: States, quad_order=1, stabilization=True, save=True,
+                 R=None, vals=None):
+        """One assembly over the parts, exported into R and vals (added to
+        what they hold).  Returns (R, vals, seconds) with seconds = the longest
+        part's (zero, element assembly, export) times."""
+        m = self.mesh
+        mats = np.ascontiguousarray(np.asarray(mats, dtype=np.float64).reshape(-1, 5))
+        x = _f64(x)
+        if R is None:
+            R = np.zeros(m.num_dofs)
+        if vals is None and kind in (K_JACOBIAN, K_ADJOINT):
+            vals = np.zeros(m.nnz)
+        sec = np.zeros(3)
+        st = lib.oracle_assemble_parts(self.h, kind, m.order, quad_order, model, int(bool(stabilization)),
+                                       int(bool(save)), _ptr(m.coords), _ptr(m.conn), _ptr(m.elem_set), _ptr(mats),
+                                       _ptr(x), _ptr(m.cols), _ptr(R), _ptr(vals), _ptr(states.Fp_old),
+                                       _ptr(states.eqps_old), _ptr(states.Fp), _ptr(states.eqps),
+                                       _ptr(states.sigma), _ptr(sec))
+        if st != 0:
+            raise RuntimeError(f"oracle_assemble_parts failed: {st}")
+        return R, vals, sec
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oracle_parts_destroy(self.h)
+            self.h = None

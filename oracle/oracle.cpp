@@ -34,6 +34,8 @@
 //   DWR transfers            src/goal_nested.cpp:369-412; src/goal_nested_adjoint.cpp:182-197
 //   error indicators         src/goal_error.cpp:7-56
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -223,6 +225,11 @@ struct Problem {
   double* sigma;
   const double* z_diff;
   const double* z_coarse;
+  // Rank-local rows (partitioned assembly): global node -> local node, and the
+  // offset of a local node's four CSR rows in the part's value array.  Null:
+  // R and vals are the global arrays.
+  const int32_t* lnode = nullptr;
+  const int64_t* loff = nullptr;
 };
 
 // quadrature (parent tet, weights sum to 1/6)
@@ -531,6 +538,18 @@ int assemble_range(const Problem& P, int kind, int64_t e0, int64_t e1, double* R
     int lids[40];
     for (int n = 0; n < nn; ++n)
       for (int q = 0; q < 4; ++q) lids[4 * n + q] = 4 * ec[n] + q;
+    // where node n's rows live: R row base and CSR value shift
+    int64_t rbase[10], vshift[10];
+    for (int n = 0; n < nn; ++n) {
+      if (P.lnode) {
+        const int32_t ln = P.lnode[ec[n]];
+        rbase[n] = 4 * (int64_t)ln;
+        vshift[n] = P.loff[ln] - P.row_ptr[4 * (int64_t)ec[n]];
+      } else {
+        rbase[n] = 4 * (int64_t)ec[n];
+        vshift[n] = 0;
+      }
+    }
     if (kind == K_JAC || kind == K_ADJ) {
       ElemOut<Fad<ND>> o;
       element(P, e, kind, o);
@@ -538,17 +557,17 @@ int assemble_range(const Problem& P, int kind, int64_t e0, int64_t e1, double* R
         for (int i = 0; i < 4; ++i) {
           const Fad<ND>& v = i < 3 ? o.ru[n][i] : o.rp[n];
           int row = lids[4 * n + i];
-          if (R) R[row] += v.val();
+          if (R) R[rbase[n] + i] += v.val();
           if (!vals) continue;
           for (int c = 0; c < 4 * nn; ++c) {
             if (kind == K_JAC) {
               int64_t slot = find_col(P, row, lids[c]);
               if (slot < 0) return 2;
-              vals[slot] += v.dx(c);
+              vals[slot + vshift[n]] += v.dx(c);
             } else {
               int64_t slot = find_col(P, lids[c], row);
               if (slot < 0) return 2;
-              vals[slot] += v.dx(c);
+              vals[slot + vshift[c >> 2]] += v.dx(c);
             }
           }
         }
@@ -556,7 +575,7 @@ int assemble_range(const Problem& P, int kind, int64_t e0, int64_t e1, double* R
       ElemOut<double> o;
       element(P, e, kind, o);
       for (int n = 0; n < nn; ++n)
-        for (int i = 0; i < 4; ++i) R[lids[4 * n + i]] += i < 3 ? o.ru[n][i] : o.rp[n];
+        for (int i = 0; i < 4; ++i) R[rbase[n] + i] += i < 3 ? o.ru[n][i] : o.rp[n];
     }
   }
   return 0;
@@ -669,6 +688,179 @@ int oracle_assemble(int kind, int order, int quad_order, int model, int stab, in
   for (auto& th : merge) th.join();
   for (int s : st)
     if (s) return s;
+  return 0;
+}
+
+// ------------------------------------------------ rank-per-part assembly
+// The reference's parallel model on one host: one part per core, each owning
+// a contiguous range of elements (a slab of the mesh, since elements are
+// numbered along the mesh) and holding only the rows of the nodes its
+// elements touch -- its owned and ghost rows, as the ghost SolInfo of one MPI
+// rank (src/goal_sol_info.cpp:6-31).  An assembly is, per part, zero its
+// rows, assemble its elements into them (scatter, src/goal_displacement.cpp:
+// 177-194), then the Export ADD of the ghost rows into their owners
+// (src/goal_sol_info.cpp:33-43): every node is owned by the first part that
+// touches it, which sums the node's rows over all touching parts in part order.
+// The plan (which rows each part holds) is built once, as the reference's
+// Disc builds its maps once per mesh.
+struct OracleParts {
+  int threads = 1;
+  int64_t nn = 0, ne = 0;
+  const int64_t* row_ptr = nullptr;
+  std::vector<int64_t> e0;                   // [threads + 1]
+  std::vector<std::vector<int32_t>> nodes;   // per part: touched nodes, ascending
+  std::vector<std::vector<int32_t>> lnode;   // per part: node -> local index or -1
+  std::vector<std::vector<int64_t>> loff;    // per part: local node -> value offset [nodes + 1]
+  std::vector<std::vector<double>> R, V;     // per part: local rows
+  std::vector<int64_t> tptr;                 // node -> touching parts (CSR)
+  std::vector<int32_t> tpart, tloc;
+  std::vector<std::vector<int32_t>> owned;   // per part: nodes it owns
+};
+
+void* oracle_parts_create(int order, int64_t nn, int64_t ne, const int32_t* conn, const int64_t* row_ptr,
+                          int threads) {
+  const int npe = order == 1 ? 4 : 10;
+  auto* P = new OracleParts;
+  P->threads = std::max(1, threads);
+  P->nn = nn;
+  P->ne = ne;
+  P->row_ptr = row_ptr;
+  const int T = P->threads;
+  P->e0.resize(T + 1);
+  for (int t = 0; t <= T; ++t) P->e0[t] = ne * t / T;
+  P->nodes.resize(T);
+  P->lnode.resize(T);
+  P->loff.resize(T);
+  P->R.resize(T);
+  P->V.resize(T);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < T; ++t)
+    pool.emplace_back([&, t] {
+      std::vector<int32_t>& ln = P->lnode[t];
+      ln.assign(nn, -1);
+      for (int64_t e = P->e0[t]; e < P->e0[t + 1]; ++e)
+        for (int k = 0; k < npe; ++k) ln[conn[e * npe + k]] = 0;
+      std::vector<int32_t>& nd = P->nodes[t];
+      for (int64_t a = 0; a < nn; ++a)
+        if (ln[a] == 0) nd.push_back((int32_t)a);
+      std::vector<int64_t>& off = P->loff[t];
+      off.assign(nd.size() + 1, 0);
+      for (size_t k = 0; k < nd.size(); ++k) {
+        ln[nd[k]] = (int32_t)k;
+        off[k + 1] = off[k] + (row_ptr[4 * (int64_t)nd[k] + 4] - row_ptr[4 * (int64_t)nd[k]]);
+      }
+      P->R[t].assign(4 * nd.size(), 0.0);
+      P->V[t].assign(off.back(), 0.0);
+    });
+  for (auto& th : pool) th.join();
+  // touching parts of every node, in part order; the first one owns it
+  P->tptr.assign(nn + 1, 0);
+  for (int t = 0; t < T; ++t)
+    for (int32_t a : P->nodes[t]) ++P->tptr[a + 1];
+  for (int64_t a = 0; a < nn; ++a) P->tptr[a + 1] += P->tptr[a];
+  P->tpart.resize(P->tptr[nn]);
+  P->tloc.resize(P->tptr[nn]);
+  std::vector<int64_t> fill(P->tptr.begin(), P->tptr.end() - 1);
+  for (int t = 0; t < T; ++t)
+    for (size_t k = 0; k < P->nodes[t].size(); ++k) {
+      const int32_t a = P->nodes[t][k];
+      P->tpart[fill[a]] = t;
+      P->tloc[fill[a]++] = (int32_t)k;
+    }
+  P->owned.resize(T);
+  for (int64_t a = 0; a < nn; ++a)
+    if (P->tptr[a + 1] > P->tptr[a]) P->owned[P->tpart[P->tptr[a]]].push_back((int32_t)a);
+  return P;
+}
+
+void oracle_parts_destroy(void* h) { delete static_cast<OracleParts*>(h); }
+
+// Rows held by each part and its ghost share: out[0] = sum of rows held over
+// parts, out[1] = rows of the mesh (a measure of the ghost overhead).
+void oracle_parts_rows(void* h, int64_t* out) {
+  auto* P = static_cast<OracleParts*>(h);
+  int64_t held = 0;
+  for (auto& nd : P->nodes) held += 4 * (int64_t)nd.size();
+  out[0] = held;
+  out[1] = 4 * P->nn;
+}
+
+// One assembly over the parts (arguments as oracle_assemble); R and vals
+// (global) receive the exported sums, added to what they hold.  seconds[3]:
+// the longest part's zeroing, element assembly and export times.
+int oracle_assemble_parts(void* h, int kind, int order, int quad_order, int model, int stab, int save,
+                          const double* coords, const int32_t* conn, const int32_t* elem_set, const double* mats,
+                          const double* x, const int32_t* cols, double* R, double* vals, const double* Fp_old,
+                          const double* eqps_old, double* Fp, double* eqps, double* sigma, double* seconds) {
+  auto* H = static_cast<OracleParts*>(h);
+  const int T = H->threads;
+  std::vector<int> st(T, 0);
+  std::vector<double> tz(T), ta(T), tx(T);
+  std::vector<std::thread> pool;
+  std::atomic<int> arrived{0};
+  for (int t = 0; t < T; ++t)
+    pool.emplace_back([&, t] {
+      using clk = std::chrono::steady_clock;
+      auto c0 = clk::now();
+      std::fill(H->R[t].begin(), H->R[t].end(), 0.0);
+      if (vals) std::fill(H->V[t].begin(), H->V[t].end(), 0.0);
+      auto c1 = clk::now();
+      Problem P;
+      P.order = order;
+      P.npe = order == 1 ? 4 : 10;
+      P.nq = quad_order == 1 ? 1 : 4;
+      P.model = model;
+      P.stab = stab;
+      P.save = save;
+      P.nn = H->nn;
+      P.ne = H->ne;
+      P.coords = coords;
+      P.conn = conn;
+      P.elem_set = elem_set;
+      P.mats = reinterpret_cast<const Mat*>(mats);
+      P.x = x;
+      P.row_ptr = H->row_ptr;
+      P.cols = cols;
+      P.Fp_old = Fp_old;
+      P.eqps_old = eqps_old;
+      P.Fp = Fp;
+      P.eqps = eqps;
+      P.sigma = sigma;
+      P.z_diff = nullptr;
+      P.z_coarse = nullptr;
+      P.lnode = H->lnode[t].data();
+      P.loff = H->loff[t].data();
+      st[t] = assemble_any(P, kind, H->e0[t], H->e0[t + 1], H->R[t].data(), vals ? H->V[t].data() : nullptr);
+      auto c2 = clk::now();
+      // every part's rows are complete before any export reads them
+      arrived.fetch_add(1);
+      while (arrived.load() < T) std::this_thread::yield();
+      auto c2b = clk::now();
+      for (int32_t a : H->owned[t]) {
+        const int64_t r0 = H->row_ptr[4 * (int64_t)a];
+        const int64_t len = H->row_ptr[4 * (int64_t)a + 4] - r0;
+        for (int64_t k = H->tptr[a]; k < H->tptr[a + 1]; ++k) {
+          const int p = H->tpart[k], l = H->tloc[k];
+          for (int i = 0; i < 4; ++i) R[4 * (int64_t)a + i] += H->R[p][4 * (int64_t)l + i];
+          if (vals) {
+            const double* src = H->V[p].data() + H->loff[p][l];
+            for (int64_t j = 0; j < len; ++j) vals[r0 + j] += src[j];
+          }
+        }
+      }
+      auto c3 = clk::now();
+      tz[t] = std::chrono::duration<double>(c1 - c0).count();
+      ta[t] = std::chrono::duration<double>(c2 - c1).count();
+      tx[t] = std::chrono::duration<double>(c3 - c2b).count();
+    });
+  for (auto& th : pool) th.join();
+  if (seconds) {
+    seconds[0] = *std::max_element(tz.begin(), tz.end());
+    seconds[1] = *std::max_element(ta.begin(), ta.end());
+    seconds[2] = *std::max_element(tx.begin(), tx.end());
+  }
+  for (int v : st)
+    if (v) return v;
   return 0;
 }
 

@@ -412,3 +412,30 @@ def test_graph_three_ways():
         rp_n, cols_n = npref.graph_bruteforce(conn.shape[1], coords.shape[0], conn)
         assert np.array_equal(rp_o, rp_n) and np.array_equal(cols_o, cols_n)
         assert np.array_equal(rp_l, rp_n) and np.array_equal(cols_l, cols_n)
+
+
+@pytest.mark.parametrize("order,quad,model", [(2, 2, 1), (1, 1, 2)])
+def test_rank_per_part_assembly_equals_serial(order, quad, model):
+    """The CPU baseline's rank-per-part assembly (each part holds only the
+    rows its elements touch, then exports ghost rows to their owners) gives
+    the serial oracle's R and CSR values up to summation order, for 1..7
+    parts, residual, Jacobian and transposed kinds; each part holds fewer rows
+    than the mesh, and together at least all of them."""
+    coords, conn = g.mesh_box(6, 5, 4, order=order, jitter=0.1, seed=3)
+    m = O.Mesh(order, coords, conn)
+    x = random_state(m.num_nodes, 4, scale_u=0.005)
+    mats = [mat_row(MODELS[model])]
+    for kind in (O.K_JACOBIAN, O.K_ADJOINT, O.K_RESIDUAL):
+        st = O.States(m, model, quad)
+        R1, V1 = O.assemble(kind, m, model, mats, x, st, quad_order=quad)
+        for T in (1, 2, 7):
+            parts = O.Parts(m, T)
+            st2 = O.States(m, model, quad)
+            R2, V2, sec = parts.assemble(kind, model, mats, x, st2, quad_order=quad)
+            assert np.linalg.norm(R2 - R1) <= 1e-14 * np.linalg.norm(R1)
+            if V1 is not None:
+                assert np.linalg.norm(V2 - V1) <= 1e-14 * np.linalg.norm(V1)
+            assert np.array_equal(st.sigma, st2.sigma)
+            held, total = parts.rows()
+            assert total == m.num_dofs and (held == total if T == 1 else held > total)
+            assert (sec >= 0).all()
