@@ -201,6 +201,31 @@ def cpu_baseline(sample, threads: int, wl, reps: int = 5):
     return m.num_elems / med, m.num_elems, med, split, (coords, conn, x, R, V)
 
 
+def accuracy(g, wl, sample):
+    """The GPU assembly of the CPU-baseline sample against the oracle's R and
+    dR/du of the same run (normwise relative), FP64 and FP32-tangent
+    element matrices."""
+    import numpy as np
+
+    coords, conn, x, R, V = sample
+    disc = g.Disc.from_arrays(wl["order"], coords, conn)
+    st = g.States(disc, wl["model"], wl["quad"])
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(model=wl["model"], **wl["mat"])], stabilization=True,
+                       quad_order=wl["quad"])
+    disc.set_soln(x)
+    si = g.SolInfo(disc)
+    out = {}
+    for tangent in (64, 32):
+        mech.set_option("tangent", tangent)
+        si.zero_all()
+        mech.jacobian(si, adjoint=False, save_states=False)
+        r, v = si.R(), si.vals()
+        tag = "" if tangent == 64 else "_fp32_tangent"
+        out["R_rel" + tag] = float(np.linalg.norm(r - R) / np.linalg.norm(R))
+        out["dRdu_rel" + tag] = float(np.linalg.norm(v - V) / np.linalg.norm(V))
+    return out
+
+
 def _read_csv(path):
     import csv
 
