@@ -129,6 +129,8 @@ _sig("goal_solinfo_dev", _i32, _p, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p))
 _sig("goal_solinfo_norm_R", _i32, _p, _dp)
 _sig("goal_solinfo_dot_R", _i32, _p, _p, _dp)
 _sig("goal_linear_solve", _i32, _p, _p, _p, C.c_double, _i32, _i32, C.POINTER(_i32), _dp)
+_sig("goal_linear_solve_pc", _i32, _p, _p, _p, C.c_double, _i32, _i32, _i32, C.POINTER(_i32), _dp)
+PRECOND_BLOCK_JACOBI, PRECOND_AMG = 0, 1
 _sig("goal_device_fill", _i32, _p, _i64, C.c_double)
 _sig("goal_device_scale", _i32, _p, _p, _i64, C.c_double)
 _sig("goal_mech_create", _i32, _p, _p, _i32, _p, _i32, _i32, C.POINTER(_p))
@@ -627,15 +629,17 @@ class SolInfo:
         check(lib.goal_solinfo_dev(self.h, C.byref(r), C.byref(m), C.byref(v)))
         return r.value, m.value, v.value
 
-    def solve(self, b, x, rtol=1e-10, max_iters=2000, restart=100):
-        """dRdu x = b on the device (GMRES, node block-Jacobi); x holds the
-        initial guess.  b, x: DeviceArray or raw device pointers.  Returns
-        (inner iterations, relative residual ||b - Ax|| / ||b - Ax0||)."""
+    def solve(self, b, x, rtol=1e-10, max_iters=2000, restart=100, precond=PRECOND_AMG):
+        """dRdu x = b on the device (GMRES preconditioned by aggregation
+        multigrid, or node block-Jacobi); x holds the initial guess.  b, x:
+        DeviceArray or raw device pointers.  Returns (inner iterations,
+        relative residual ||b - Ax|| / ||b - Ax0||)."""
         it = C.c_int(0)
         rr = C.c_double(0.0)
         bp = b.ptr if isinstance(b, DeviceArray) else b
         xp = x.ptr if isinstance(x, DeviceArray) else x
-        check(lib.goal_linear_solve(self.h, bp, xp, rtol, max_iters, restart, C.byref(it), C.byref(rr)))
+        check(lib.goal_linear_solve_pc(self.h, bp, xp, rtol, max_iters, restart, precond, C.byref(it),
+                                       C.byref(rr)))
         return it.value, rr.value
 
     def norm_R(self) -> float:

@@ -53,6 +53,7 @@ SOURCES = [
     "goal_adapt.cpp",
     "goal_meshio.cpp",
     "goal_expr.cpp",
+    "goal_amg.cpp",
     "goal_capi.cpp",
     "goal_assembly.hip",
     "goal_util.hip",

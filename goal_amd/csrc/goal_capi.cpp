@@ -20,6 +20,7 @@
 #include <thread>
 #include <vector>
 
+#include "goal_amg.hpp"
 #include "goal_expr.hpp"
 #include "goal_kernels.hpp"
 #include "goal_mesh.hpp"
@@ -197,9 +198,33 @@ goal::MatParams derive(const goal_material& m) {
 }  // namespace
 
 // ---------------------------------------------------------------- handles
+// Aggregation multigrid over a disc's node-block matrices (the preconditioner
+// of goal_linear_solve).  Structure once per disc; values for every solve.
+struct AmgLevel {
+  int64_t nn = 0;                         // nodes (4 equations each)
+  DevBuf<int64_t> bptr;                   // coarse levels: node-block CSR
+  DevBuf<int32_t> bcol;
+  DevBuf<double> bval;
+  std::vector<int64_t> bptr_h;            // the coarsest level's graph, for its dense inverse
+  std::vector<int32_t> bcol_h;
+  int64_t nc = 0, ncb = 0;                // transfer to the next level
+  DevBuf<int32_t> agg, amem;
+  DevBuf<int64_t> aptr, gptr, gidx;
+  DevBuf<double> binv, b, x, r, d;        // smoother inverse blocks, work vectors
+  double lmax = 1.0;                      // spectral radius estimate of Binv A
+};
+struct AmgHierarchy {
+  std::vector<std::unique_ptr<AmgLevel>> lv;
+  DevBuf<int32_t> brow0;                  // row node of every finest-level block
+  DevBuf<double> coarse_inv;              // dense inverse of the coarsest level
+  int coarse_n = 0;
+};
+
 struct goal_disc_s {
   goal::HostMesh mesh;
   std::vector<int64_t> node_adj_ptr;
+  std::vector<int32_t> node_adj;
+  std::unique_ptr<AmgHierarchy> amg;
   int max_degree = 0;
   int64_t num_dofs = 0, nnz = 0;
   // gather plan: runs of consecutive nodes whose CSR values fit `gather_cap`
@@ -299,6 +324,7 @@ namespace {
 void disc_build(goal_disc d) {
   goal::HostGraph g = goal::build_graph(d->mesh, host_threads());
   d->node_adj_ptr = g.node_adj_ptr;
+  d->amg.reset();
   d->max_degree = g.max_degree;
   d->num_dofs = g.num_dofs;
   d->nnz = g.nnz;
@@ -335,6 +361,7 @@ void disc_build(goal_disc d) {
   DevBuf<int32_t> adj;
   adj_ptr.upload(g.node_adj_ptr.data(), (int64_t)g.node_adj_ptr.size());
   adj.upload(g.node_adj.data(), (int64_t)g.node_adj.size());
+  d->node_adj = std::move(g.node_adj);
   d->row_ptr.alloc(d->num_dofs + 1);
   d->cols.alloc(d->nnz);
   HIPCHK(goal::launch_expand_csr(adj_ptr.p, adj.p, m.num_nodes, d->row_ptr.p, d->cols.p, stream()));
@@ -574,6 +601,195 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
   }
   s->zero_R = false;
   if (want_vals) s->zero_vals = false;
+}
+
+// ------------------------------------------------- multigrid preconditioner
+// Aggregation multigrid (one V-cycle per application) standing in for the
+// MueLu preconditioner of goal's solve (src/goal_linear_solve.cpp:53-70).
+//   - Levels: greedy aggregates of nodes (goal::amg_coarsen), the prolongator
+//     P the aggregates' indicator on each of a node's 4 equations, coarse
+//     matrices the Galerkin sums P^T A P, down to at most kAmgCoarseNodes nodes,
+//     whose matrix is inverted densely.
+//   - Smoother: Chebyshev of degree kAmgDegree on Binv A (Binv: inverse 4x4
+//     diagonal blocks), on [lmax / 30, 1.1 lmax] with lmax from a power
+//     iteration, before and after the coarse correction.
+// Every kernel sums in a fixed order, so a solve is bitwise reproducible.
+constexpr int kAmgCoarseNodes = 128;
+constexpr int kAmgMaxLevels = 12;
+constexpr int kAmgDegree = 3;
+constexpr int kAmgPowerIters = 12;
+
+void amg_build_structure(goal_disc d) {
+  auto h = std::make_unique<AmgHierarchy>();
+  const int64_t nn0 = d->mesh.num_nodes;
+  std::vector<int64_t> fptr = d->node_adj_ptr;
+  std::vector<int32_t> fcol = d->node_adj;
+  int64_t nf = nn0;
+  {  // row node of every finest-level block
+    std::vector<int32_t> brow(fptr[nf]);
+    for (int64_t a = 0; a < nf; ++a)
+      for (int64_t k = fptr[a]; k < fptr[a + 1]; ++k) brow[k] = (int32_t)a;
+    h->brow0.upload(brow.data(), (int64_t)brow.size());
+    sync();
+  }
+  for (int l = 0;; ++l) {
+    auto L = std::make_unique<AmgLevel>();
+    L->nn = nf;
+    if (l > 0) {
+      L->bptr.upload(fptr.data(), (int64_t)fptr.size());
+      L->bcol.upload(fcol.data(), (int64_t)fcol.size());
+      L->bval.alloc(16 * fptr[nf]);
+    }
+    L->binv.alloc(16 * nf);
+    L->b.alloc(4 * nf);
+    L->x.alloc(4 * nf);
+    L->r.alloc(4 * nf);
+    L->d.alloc(4 * nf);
+    const bool last = nf <= kAmgCoarseNodes || l + 1 >= kAmgMaxLevels;
+    goal::AmgCoarsening c;
+    if (!last) c = goal::amg_coarsen(nf, fptr.data(), fcol.data());
+    if (last || c.nc >= nf) {  // coarsest: keep its graph for the dense inverse
+      L->bptr_h = fptr;
+      L->bcol_h = fcol;
+      h->lv.push_back(std::move(L));
+      sync();
+      break;
+    }
+    L->nc = c.nc;
+    L->ncb = (int64_t)c.ccol.size();
+    L->agg.upload(c.agg.data(), (int64_t)c.agg.size());
+    L->aptr.upload(c.aptr.data(), (int64_t)c.aptr.size());
+    L->amem.upload(c.amem.data(), (int64_t)c.amem.size());
+    L->gptr.upload(c.gptr.data(), (int64_t)c.gptr.size());
+    L->gidx.upload(c.gidx.data(), (int64_t)c.gidx.size());
+    sync();  // the host vectors go out of scope
+    h->lv.push_back(std::move(L));
+    fptr.swap(c.cptr);
+    fcol.swap(c.ccol);
+    nf = c.nc;
+  }
+  const int64_t nc = h->lv.back()->nn;
+  h->coarse_n = (int)(4 * nc);
+  h->coarse_inv.alloc((int64_t)h->coarse_n * h->coarse_n);
+  d->amg = std::move(h);
+}
+
+// y = b - A_l x, or y = A_l x without b (finest level: the solve's CSR).
+void amg_residual(goal_solinfo s, AmgHierarchy& h, int l, const double* x, const double* b, double* y) {
+  AmgLevel& L = *h.lv[l];
+  goal_disc d = s->d;
+  if (l == 0) {
+    if (b) HIPCHK(goal::launch_spmv_resid(4 * L.nn, d->row_ptr.p, d->cols.p, s->vals.p, x, b, y, stream()));
+    else HIPCHK(goal::launch_spmv(4 * L.nn, d->row_ptr.p, d->cols.p, s->vals.p, x, y, stream()));
+  } else {
+    HIPCHK(goal::launch_bsr_spmv(L.nn, L.bptr.p, L.bcol.p, L.bval.p, x, b, y, stream()));
+  }
+}
+
+// Values of the hierarchy for the solve's current matrix.
+void amg_setup(goal_solinfo s, int* status) {
+  goal_disc d = s->d;
+  if (!d->amg) amg_build_structure(d);
+  AmgHierarchy& h = *d->amg;
+  const int nl = (int)h.lv.size();
+  for (int l = 0; l < nl; ++l) {
+    AmgLevel& L = *h.lv[l];
+    if (l == 0) {
+      HIPCHK(goal::launch_block_inverse(L.nn, d->row_ptr.p, d->cols.p, s->vals.p, L.binv.p, status, stream()));
+    } else {
+      AmgLevel& F = *h.lv[l - 1];
+      HIPCHK(goal::launch_galerkin(F.ncb, F.gptr.p, F.gidx.p, l == 1 ? h.brow0.p : nullptr, d->row_ptr.p,
+                                   l == 1 ? s->vals.p : F.bval.p, L.bval.p, stream()));
+      HIPCHK(goal::launch_bsr_block_inverse(L.nn, L.bptr.p, L.bcol.p, L.bval.p, L.binv.p, status, stream()));
+    }
+    if (l + 1 == nl) break;
+    // spectral radius of Binv A by power iteration from a fixed start vector
+    std::vector<double> v0(4 * L.nn);
+    for (int64_t i = 0; i < 4 * L.nn; ++i)
+      v0[i] = (double)(goal::splitmix64(0xA3C5u + (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    HIPCHK(hipMemcpyAsync(L.x.p, v0.data(), sizeof(double) * v0.size(), hipMemcpyHostToDevice, stream()));
+    double lam = 1.0;
+    for (int it = 0; it < kAmgPowerIters; ++it) {
+      amg_residual(s, h, l, L.x.p, nullptr, L.r.p);
+      HIPCHK(goal::launch_block_apply(L.nn, L.binv.p, L.r.p, L.d.p, stream()));
+      HIPCHK(goal::launch_reduce(1, L.d.p, nullptr, 4 * L.nn, reduce_scratch(), result_slot(3), stream()));
+      HIPCHK(goal::launch_reduce(1, L.x.p, nullptr, 4 * L.nn, reduce_scratch(), result_slot(4), stream()));
+      double nd = 0.0, nx = 0.0;
+      HIPCHK(hipMemcpyAsync(&nd, result_slot(3), sizeof(double), hipMemcpyDeviceToHost, stream()));
+      HIPCHK(hipMemcpyAsync(&nx, result_slot(4), sizeof(double), hipMemcpyDeviceToHost, stream()));
+      sync();
+      if (!(nd > 0.0) || !(nx > 0.0)) break;
+      lam = std::sqrt(nd / nx);
+      HIPCHK(goal::launch_scale(4 * L.nn, 1.0 / std::sqrt(nd), L.d.p, L.x.p, stream()));
+    }
+    L.lmax = lam;
+  }
+  // the coarsest level's matrix, densely inverted on the host
+  AmgLevel& C = *h.lv[nl - 1];
+  const int n = h.coarse_n;
+  std::vector<double> dense((size_t)n * n, 0.0);
+  if (nl == 1) {  // the whole problem is the coarsest level: its CSR
+    std::vector<double> vals(s->vals.n);
+    std::vector<int64_t> rp(d->row_ptr.n);
+    std::vector<int32_t> cols(d->cols.n);
+    s->vals.download(vals.data(), s->vals.n);
+    d->row_ptr.download(rp.data(), d->row_ptr.n);
+    d->cols.download(cols.data(), d->cols.n);
+    for (int r = 0; r < n; ++r)
+      for (int64_t k = rp[r]; k < rp[r + 1]; ++k) dense[(size_t)r * n + cols[k]] = vals[k];
+  } else {
+    std::vector<double> bv(C.bval.n);
+    C.bval.download(bv.data(), C.bval.n);
+    for (int64_t a = 0; a < C.nn; ++a)
+      for (int64_t k = C.bptr_h[a]; k < C.bptr_h[a + 1]; ++k)
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j)
+            dense[(size_t)(4 * a + i) * n + 4 * (int64_t)C.bcol_h[k] + j] = bv[16 * k + 4 * i + j];
+  }
+  if (!goal::dense_inverse(n, dense)) {
+    int one = 1;
+    HIPCHK(hipMemcpyAsync(status, &one, sizeof(int), hipMemcpyHostToDevice, stream()));
+  }
+  HIPCHK(hipMemcpyAsync(h.coarse_inv.p, dense.data(), sizeof(double) * dense.size(), hipMemcpyHostToDevice, stream()));
+  sync();
+}
+
+// x = Chebyshev smoothing of A_l x = b (from x = 0 when zero).
+void amg_smooth(goal_solinfo s, AmgHierarchy& h, int l, const double* b, double* x, bool zero) {
+  AmgLevel& L = *h.lv[l];
+  const double hi = 1.1 * L.lmax, lo = hi / 30.0;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  const double* r = b;
+  if (zero) {
+    HIPCHK(goal::launch_fill(x, 4 * L.nn, 0.0, stream()));
+  } else {
+    amg_residual(s, h, l, x, b, L.r.p);
+    r = L.r.p;
+  }
+  HIPCHK(goal::launch_cheb(L.nn, L.binv.p, r, 0.0, 1.0 / theta, L.d.p, x, stream()));
+  for (int k = 1; k < kAmgDegree; ++k) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    amg_residual(s, h, l, x, b, L.r.p);
+    HIPCHK(goal::launch_cheb(L.nn, L.binv.p, L.r.p, rn * rho, 2.0 * rn / delta, L.d.p, x, stream()));
+    rho = rn;
+  }
+}
+
+// x = V-cycle applied to b on level l.
+void amg_vcycle(goal_solinfo s, AmgHierarchy& h, int l, const double* b, double* x) {
+  AmgLevel& L = *h.lv[l];
+  if (l + 1 == (int)h.lv.size()) {
+    HIPCHK(goal::launch_dense_matvec(h.coarse_n, h.coarse_inv.p, b, x, stream()));
+    return;
+  }
+  AmgLevel& C = *h.lv[l + 1];
+  amg_smooth(s, h, l, b, x, true);
+  amg_residual(s, h, l, x, b, L.r.p);
+  HIPCHK(goal::launch_restrict(L.nc, L.aptr.p, L.amem.p, L.r.p, C.b.p, stream()));
+  amg_vcycle(s, h, l + 1, C.b.p, C.x.p);
+  HIPCHK(goal::launch_prolong(L.nn, L.agg.p, C.x.p, x, stream()));
+  amg_smooth(s, h, l, b, x, false);
 }
 
 }  // namespace
@@ -1380,9 +1596,16 @@ int goal_solinfo_dev(goal_solinfo s, double** R, double** dMdu, double** vals) {
 
 int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, int max_iters, int restart,
                       int* iters_out, double* relres_out) {
+  return goal_linear_solve_pc(s, b, x, rtol, max_iters, restart, GOAL_PRECOND_AMG, iters_out, relres_out);
+}
+
+int goal_linear_solve_pc(goal_solinfo s, const double* b, double* x, double rtol, int max_iters, int restart,
+                         int precond, int* iters_out, double* relres_out) {
   return guard([&]() -> int {
     require(s && b && x && rtol > 0.0 && max_iters >= 0 && restart >= 1 && restart <= 1000,
             "goal_linear_solve: bad arguments");
+    require(precond == GOAL_PRECOND_AMG || precond == GOAL_PRECOND_BLOCK_JACOBI,
+            "goal_linear_solve: unknown preconditioner");
     materialize(s);
     goal_disc d = s->d;
     const int64_t n = d->num_dofs, nn = d->mesh.num_nodes;
@@ -1404,7 +1627,13 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
     double* V = s->V.p;
     double* t = s->t.p;
     double* z = s->z.p;
-    HIPCHK(goal::launch_block_inverse(nn, rp, cols, A, s->binv.p, s->solve_status.p, stream()));
+    const bool amg = precond == GOAL_PRECOND_AMG;
+    if (amg) amg_setup(s, s->solve_status.p);
+    else HIPCHK(goal::launch_block_inverse(nn, rp, cols, A, s->binv.p, s->solve_status.p, stream()));
+    auto prec = [&](const double* in, double* out) {  // out = M^{-1} in
+      if (amg) amg_vcycle(s, *d->amg, 0, in, out);
+      else HIPCHK(goal::launch_block_apply(nn, s->binv.p, in, out, stream()));
+    };
     auto norm2 = [&](const double* v) {  // waits
       HIPCHK(goal::launch_reduce(1, v, nullptr, n, reduce_scratch(), result_slot(2), stream()));
       double r = 0.0;
@@ -1432,7 +1661,7 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
       bool done = false;
       while (j < m && total < max_iters) {
         double* w = V + (int64_t)(j + 1) * n;
-        HIPCHK(goal::launch_block_apply(nn, s->binv.p, V + (int64_t)j * n, t, stream()));
+        prec(V + (int64_t)j * n, t);
         HIPCHK(goal::launch_spmv(n, rp, cols, A, t, w, stream()));
         // classical Gram-Schmidt with the DGKS test (the reference's Belos
         // orthogonalization, src/goal_linear_solve.cpp:53-90): a second pass
@@ -1483,7 +1712,7 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
         HIPCHK(hipMemcpyAsync(s->y.p, yv.data(), sizeof(double) * j, hipMemcpyHostToDevice, stream()));
         HIPCHK(goal::launch_fill(z, n, 0.0, stream()));
         HIPCHK(goal::launch_multi_axpy(n, j, V, n, s->y.p, 1.0, z, stream()));
-        HIPCHK(goal::launch_block_apply(nn, s->binv.p, z, t, stream()));
+        prec(z, t);
         HIPCHK(goal::launch_axpy(x, t, 1.0, n, stream()));
         sync();  // yv is reused by the next cycle
       }
@@ -1494,7 +1723,7 @@ int goal_linear_solve(goal_solinfo s, const double* b, double* x, double rtol, i
     sync();
     if (iters_out) *iters_out = total;
     if (relres_out) *relres_out = relres;
-    if (st) return set_error(GOAL_ERR_INVALID, "goal_linear_solve: singular 4x4 diagonal block");
+    if (st) return set_error(GOAL_ERR_INVALID, "goal_linear_solve: singular diagonal block or coarsest matrix");
     return GOAL_OK;
   });
 }

@@ -118,6 +118,21 @@ hipError_t launch_spmv(int64_t nrows, const int64_t* rp, const int32_t* cols, co
 hipError_t launch_block_inverse(int64_t nn, const int64_t* rp, const int32_t* cols, const double* vals,
                                 double* binv, int* status, hipStream_t s);
 hipError_t launch_block_apply(int64_t nn, const double* binv, const double* x, double* y, hipStream_t s);
+// aggregation multigrid: BSR (4x4 node blocks) levels, Galerkin sums, transfers, smoother
+hipError_t launch_bsr_block_inverse(int64_t nb, const int64_t* bptr, const int32_t* bcol, const double* bval,
+                                    double* binv, int* status, hipStream_t s);
+hipError_t launch_bsr_spmv(int64_t nb, const int64_t* bptr, const int32_t* bcol, const double* bval, const double* x,
+                           const double* b, double* y, hipStream_t s);  // y = A x, or b - A x with b
+hipError_t launch_spmv_resid(int64_t nrows, const int64_t* rp, const int32_t* cols, const double* vals,
+                             const double* x, const double* b, double* y, hipStream_t s);  // y = b - A x
+hipError_t launch_galerkin(int64_t ncb, const int64_t* gptr, const int64_t* gidx, const int32_t* brow,
+                           const int64_t* rp, const double* fvals, double* cvals, hipStream_t s);
+hipError_t launch_restrict(int64_t nc, const int64_t* aptr, const int32_t* amem, const double* r, double* bc,
+                           hipStream_t s);
+hipError_t launch_prolong(int64_t nf, const int32_t* agg, const double* xc, double* x, hipStream_t s);
+hipError_t launch_cheb(int64_t nn, const double* binv, const double* r, double c1, double c2, double* d, double* x,
+                       hipStream_t s);
+hipError_t launch_dense_matvec(int n, const double* M, const double* x, double* y, hipStream_t s);
 // out[j] = V_j . w (j < k); partial holds k * multi_dot_blocks(n) doubles
 int multi_dot_blocks(int64_t n);
 hipError_t launch_multi_dot(int64_t n, int k, const double* V, int64_t ldv, const double* w, double* partial,

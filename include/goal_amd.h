@@ -211,15 +211,23 @@ int goal_solinfo_dot_R(goal_solinfo s, const double* z_dev, double* dot);
 /* Linear solve dRdu x = b on the SolInfo's CSR values, standing in for
  * goal::solve (src/goal_linear_solve.cpp:22-90: Belos block GMRES, block
  * size 1, "krylov size" = restart, DGKS orthogonalization, left MueLu
- * preconditioner).  Here: GMRES(restart) preconditioned on the right by the
- * inverse 4x4 diagonal block of every node, classical Gram-Schmidt with the
- * DGKS test (a second pass when the first removes more than 1 - 1/sqrt(2) of
- * the norm).  b, x are device vectors of num_dofs; x holds the initial guess on
- * entry.  Stops when ||b - A x|| <= rtol ||b - A x0|| or after max_iters inner
- * iterations; like the reference it returns normally when the limit is hit
- * (*relres tells).  Outputs may be NULL. */
+ * preconditioner with "number of equations" 4).  Here: GMRES(restart)
+ * preconditioned on the right by one V-cycle of aggregation multigrid over
+ * the 4-equation node blocks (aggregates of nodes, Galerkin coarse matrices,
+ * Chebyshev smoothing on the inverse diagonal blocks, a dense inverse on the
+ * coarsest level), classical Gram-Schmidt with the DGKS test (a second pass
+ * when the first removes more than 1 - 1/sqrt(2) of the norm).  b, x are
+ * device vectors of num_dofs; x holds the initial guess on entry.  Stops when
+ * ||b - A x|| <= rtol ||b - A x0|| or after max_iters inner iterations; like
+ * the reference it returns normally when the limit is hit (*relres tells).
+ * Outputs may be NULL. */
 int goal_linear_solve(goal_solinfo s, const double* b_dev, double* x_dev, double rtol, int max_iters, int restart,
                       int* iters, double* relres);
+/* The same with the preconditioner named: GOAL_PRECOND_AMG (goal_linear_solve's)
+ * or GOAL_PRECOND_BLOCK_JACOBI (the inverse 4x4 diagonal block of every node). */
+enum { GOAL_PRECOND_BLOCK_JACOBI = 0, GOAL_PRECOND_AMG = 1 };
+int goal_linear_solve_pc(goal_solinfo s, const double* b_dev, double* x_dev, double rtol, int max_iters,
+                         int restart, int precond, int* iters, double* relres);
 
 /* ------------------------------------------------------------- mechanics */
 typedef struct goal_material {

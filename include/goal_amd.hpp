@@ -431,19 +431,23 @@ inline void set_tbcs(std::vector<TractionBC> const& bcs, SolInfo* s, double t, c
 // Return 0 on success.
 using LinearSolve = std::function<int(Disc* d, SolInfo* s, const double* b_dev, double* x_dev)>;
 
-// The library's device solve as a LinearSolve: GMRES(restart) with a node
-// block-Jacobi preconditioner from a zero initial guess, the parameters of
-// goal's "linear solve" list ("tolerance", "max iters", "krylov size";
+// The library's device solve as a LinearSolve: GMRES(restart) preconditioned
+// by aggregation multigrid over the 4-equation node blocks (as goal's MueLu
+// with "number of equations" 4; GOAL_PRECOND_BLOCK_JACOBI for the inverse
+// diagonal blocks alone) from a zero initial guess, the parameters of goal's
+// "linear solve" list ("tolerance", "max iters", "krylov size";
 // src/goal_linear_solve.cpp:22-50).  Like the reference it does not fail when
 // the iteration limit is reached.
 struct GmresSolve {
   double tolerance = 1e-10;
   int max_iters = 1000;
   int krylov_size = 100;
+  int preconditioner = GOAL_PRECOND_AMG;
   int operator()(Disc* d, SolInfo* s, const double* b_dev, double* x_dev) const {
     int st = goal_device_fill(x_dev, d->get_num_dofs(), 0.0);
     if (st != GOAL_OK) return st;
-    return goal_linear_solve(s->handle(), b_dev, x_dev, tolerance, max_iters, krylov_size, nullptr, nullptr);
+    return goal_linear_solve_pc(s->handle(), b_dev, x_dev, tolerance, max_iters, krylov_size, preconditioner,
+                                nullptr, nullptr);
   }
 };
 
