@@ -53,7 +53,7 @@ def workload(name: str, n: int, order: int):
         return dict(name=name, dims=(384, 48, 48), lengths=(8.0, 1.0, 1.0), order=1, quad=1, model=2, mat=J2_MAT,
                     b_elem=B_ELEM_P1 + B_STATE_J2, npts=1, metric="P1-tet J2 resid+Jac assembled elems/s",
                     desc="bar [0,8]x[0,1]^2 as 384x48x48 hex, jitter 10% h, J2 E=200 nu=0.3 Y=0.25 K=2 c0=1, "
-                         "1-pt quadrature, FP64 per-point history, iterate at 1% axial strain (plastic)")
+                         "1-pt quadrature, FP64 per-point history, 1% axial strain (plastic)")
     raise ValueError(name)
 
 
@@ -116,6 +116,57 @@ def newton_iterate(g, disc, mech, si, t=1.0, tol=1e-10, max_newton=4, lin_rtol=1
             "norms": [float(f"{v:.4g}") for v in norms], "seconds": round(time.perf_counter() - t0, 1)}
 
 
+BAR_BCS = [(0, "xmin", "0.0"), (1, "ymin", "0.0"), (2, "zmin", "0.0"), (0, "xmax", "0.004*t")]
+
+
+def bar_load_path(g, disc, mech, si, st, steps=20, tol=1e-10, max_newton=8, lin_rtol=1e-8, lin_max=4000):
+    """The bar workload's tension load path: rollers on xmin/ymin/zmin, u_x =
+    0.004 t on xmax for t = 1..steps (1% strain at t = 20), Newton on every
+    step with the library's device assembly, Dirichlet rows and multigrid
+    GMRES, the history updated between steps (States::update).  The timed
+    assemblies then read the last step's plastic state."""
+    import numpy as np
+
+    n = disc.num_dofs
+    disc.set_soln(np.zeros(n))
+    b, du = g.DeviceArray(n), g.DeviceArray(n)
+    t0 = time.perf_counter()
+    per_step = []
+    for step in range(1, steps + 1):
+        t = float(step)
+        its, lin, norm = 0, [], None
+        while its < max_newton:
+            its += 1
+            si.zero_all()
+            mech.jacobian(si, adjoint=False, save_states=True)
+            for comp, ns, expr in BAR_BCS:
+                g.dbc_apply(disc, si, comp, ns, expr, t, True)
+            R, _, _ = si.dev()
+            g.check(g.lib.goal_device_scale(b.ptr, R, n, -1.0))
+            du.fill(0.0)
+            k, _ = si.solve(b, du, rtol=lin_rtol, max_iters=lin_max, restart=100)
+            lin.append(k)
+            disc.add_soln_dev(du)
+            si.zero_R()
+            mech.residual(si, save_states=True)
+            for comp, ns, expr in BAR_BCS:
+                g.dbc_apply(disc, si, comp, ns, expr, t, False)
+            norm = si.norm_R()
+            if norm < tol:
+                break
+        eq = st.get("eqps")
+        per_step.append({"t": step, "newton_iters": its, "gmres_iters": lin, "norm": float(f"{norm:.4g}"),
+                         "plastic_points": int((eq > 0).sum())})
+        log(f"load step {step}: {its} Newton iterations, GMRES {lin}, ||R|| = {norm:.3e}, "
+            f"{per_step[-1]['plastic_points']} plastic points ({time.perf_counter() - t0:.1f} s)")
+        if norm >= tol:
+            raise RuntimeError(f"bar load step {step}: Newton did not converge (||R|| = {norm:.3e})")
+        if step < steps:
+            st.update()
+    return {"kind": "load_path", "steps": steps, "per_step": per_step, "seconds": round(time.perf_counter() - t0, 1),
+            "qoi_avg_disp": mech.qoi(0)}
+
+
 def smooth_state(coords, amp=0.02):
     import numpy as np
 
@@ -147,7 +198,7 @@ def host_cores():
     return use, {"nproc": os.cpu_count(), "affinity": avail, "omp_num_threads": omp or None, "cpu_model": model}
 
 
-def sample_of(disc, wl, layers):
+def sample_of(disc, wl, layers, st=None):
     """The first `layers` hexahedron layers (lowest z) of the workload's own
     mesh as a mesh of their own, with the workload's iterate on its nodes.
     Elements are numbered layer by layer (6 tets per hexahedron, x fastest),
@@ -162,7 +213,11 @@ def sample_of(disc, wl, layers):
     remap = np.full(disc.num_nodes, -1, dtype=np.int32)
     remap[used] = np.arange(used.size, dtype=np.int32)
     x = disc.soln().reshape(-1, 4)[used].ravel()
-    return coords[used], remap[conn[:ne]], x
+    hist = None
+    if st is not None and wl["model"] == 2:  # the J2 history of the sample's points (an element prefix)
+        npts = ne * wl["npts"]
+        hist = (np.ascontiguousarray(st.get("Fp_old")[:, :npts]), np.ascontiguousarray(st.get("eqps_old")[:, :npts]))
+    return coords[used], remap[conn[:ne]], x, hist
 
 
 def cpu_baseline(sample, threads: int, wl, reps: int = 5):
@@ -175,11 +230,14 @@ def cpu_baseline(sample, threads: int, wl, reps: int = 5):
 
     import oracle as O
 
-    coords, conn, x = sample
+    coords, conn, x, hist = sample
     order, quad = wl["order"], wl["quad"]
     m = O.Mesh(order, coords, conn)
     parts = O.Parts(m, threads)
     st = O.States(m, wl["model"], quad)
+    if hist is not None:
+        st.Fp_old[:] = hist[0]
+        st.eqps_old[:] = hist[1].ravel()
     mt = wl["mat"]
     mats = [[mt["E"], mt["nu"], mt["K"], mt["Y"], mt["c0"]]]
     R = np.zeros(m.num_dofs)
@@ -198,7 +256,7 @@ def cpu_baseline(sample, threads: int, wl, reps: int = 5):
     held, total = parts.rows()
     split = {"zero_s": round(float(sec[0]), 4), "elements_s": round(float(sec[1]), 4),
              "export_s": round(float(sec[2]), 4), "rows_held_over_rows": round(held / total, 3)}
-    return m.num_elems / med, m.num_elems, med, split, (coords, conn, x, R, V)
+    return m.num_elems / med, m.num_elems, med, split, (coords, conn, x, hist, R, V)
 
 
 def accuracy(g, wl, sample):
@@ -207,9 +265,12 @@ def accuracy(g, wl, sample):
     element matrices."""
     import numpy as np
 
-    coords, conn, x, R, V = sample
+    coords, conn, x, hist, R, V = sample
     disc = g.Disc.from_arrays(wl["order"], coords, conn)
     st = g.States(disc, wl["model"], wl["quad"])
+    if hist is not None:
+        st.set("Fp_old", hist[0])
+        st.set("eqps_old", hist[1])
     mech = g.Mechanics(disc, st, [g.MaterialSpec(model=wl["model"], **wl["mat"])], stabilization=True,
                        quad_order=wl["quad"])
     disc.set_soln(x)
@@ -679,8 +740,9 @@ def main() -> int:
     ap.add_argument("--problems", default="cube,bar", help="learned: which evaluation problems to run")
     ap.add_argument("--blocks", type=int, default=0)
     ap.add_argument("--iterate", default="newton", choices=("synthetic", "newton"),
-                    help="cube: the assembled state -- a smooth synthetic field, or load step 1 solved by Newton "
-                         "on the device (untimed)")
+                    help="the assembled state -- a smooth synthetic field, or solved by Newton on the device "
+                         "(untimed): the cube's load step 1, the bar's whole tension load path")
+    ap.add_argument("--load-steps", type=int, default=20, help="bar: tension load steps before timing")
     ap.add_argument("--overlap", type=int, default=-1,
                     help="neo-Hookean: assemble in this many chunks, gathering finished chunks on a second stream "
                          "(-1: the library's automatic choice, 0: off)")
@@ -752,6 +814,8 @@ def main() -> int:
     iterate = {"kind": "synthetic"}
     if args.iterate == "newton" and wl["name"] == "cube":
         iterate = newton_iterate(g, disc, mech, si)
+    elif args.iterate == "newton":
+        iterate = bar_load_path(g, disc, mech, si, st, steps=args.load_steps)
     else:
         disc.set_soln(smooth_state(coords) if wl["name"] == "cube" else bar_state(coords))
 
@@ -857,7 +921,7 @@ def main() -> int:
         layers = args.cpu_layers
         log(f"cpu baseline: oracle on the first {layers} hex layers of this mesh at this iterate, {threads} parts "
             f"({host})")
-        sample = sample_of(disc, wl, layers)
+        sample = sample_of(disc, wl, layers, st)
         v, ne_s, med, split, sample = cpu_baseline(sample, threads, wl)
         acc = accuracy(g, wl, sample)
         acc["bar_fp64"] = 1e-11
@@ -895,6 +959,8 @@ def main() -> int:
         "dtype": "f64" if args.tangent == 64 else "f64 residual, f32 element matrices",
         "data": ("synthetic (Kuhn-tet box mesh; the iterate is load step 1 solved by device Newton)"
                  if iterate["kind"] == "newton" else
+                 "synthetic (Kuhn-tet box mesh; the iterate is the last of 20 tension load steps solved by "
+                 "device Newton, with its plastic history)" if iterate["kind"] == "load_path" else
                  "synthetic (Kuhn-tet box mesh, smooth synthetic displacement/pressure iterate)"),
         "config": {
             "workload": f"{wl['desc']} -> {disc.num_elems} P{order} tets, {disc.num_dofs} dofs, nnz {disc.nnz}; "

@@ -27,14 +27,44 @@ def dbcs(coords, t):
             (2, zmax, np.full(zmax.size, 0.01 * t))]
 
 
+# The bar of the J2 tension workload: [0,8]x[0,1]^2, rollers on xmin/ymin/zmin
+# and u_x = BAR_RATE t on xmax (J2_uniaxial_3D.yaml's boundary conditions,
+# example/primal/J2_uniaxial_3D.yaml:9-13, scaled to 1% strain at t = 20).
+BAR_LENGTHS = (8.0, 1.0, 1.0)
+BAR_RATE = 0.004
+BAR_BCS = [(0, "xmin", "0.0"), (1, "ymin", "0.0"), (2, "zmin", "0.0"), (0, "xmax", "0.004*t")]
+CUBE_BCS = [(0, "zmin", "0.0"), (1, "zmin", "0.0"), (2, "zmin", "0.0"), (2, "zmax", "0.01*t")]
+
+
+def bar_dbcs(coords, t):
+    xmin = face_nodes(coords, 0, 0.0)
+    ymin = face_nodes(coords, 1, 0.0)
+    zmin = face_nodes(coords, 2, 0.0)
+    xmax = face_nodes(coords, 0, BAR_LENGTHS[0])
+    return [(0, xmin, np.zeros(xmin.size)), (1, ymin, np.zeros(ymin.size)), (2, zmin, np.zeros(zmin.size)),
+            (0, xmax, np.full(xmax.size, BAR_RATE * t))]
+
+
+def problem_mesh(problem, n, order, jitter, seed):
+    """(coords, conn, dims, lengths) of the cube (n^3) or bar (n = (nx, ny, nz)) problem."""
+    if problem == "bar":
+        dims = tuple(n)
+        coords, conn = g.mesh_box(*dims, *BAR_LENGTHS, order=order, jitter=jitter, seed=seed)
+        return coords, conn, dims, BAR_LENGTHS
+    coords, conn = g.mesh_box(n, n, n, order=order, jitter=jitter, seed=seed)
+    return coords, conn, (n, n, n), (1.0, 1.0, 1.0)
+
+
 def csr(m, vals):
     return sp.csr_matrix((vals, m.cols, m.row_ptr), shape=(m.num_dofs, m.num_dofs))
 
 
 def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0.1, seed=11, dwr=True,
-                 threads=1):
-    """Oracle Newton load steps (and, for P1, one nested-adjoint estimate)."""
-    coords, conn = g.mesh_box(n, n, n, order=order, jitter=jitter, seed=seed)
+                 threads=1, problem="cube"):
+    """Oracle Newton load steps (and, for P1, one nested-adjoint estimate) of
+    the cube (n hexahedra a side) or the bar (n = (nx, ny, nz))."""
+    coords, conn, _, _ = problem_mesh(problem, n, order, jitter, seed)
+    bcs_of = bar_dbcs if problem == "bar" else dbcs
     m = O.Mesh(order, coords, conn)
     mats = [NH_ROW if model == 1 else J2_ROW]
     st = O.States(m, model, quad)
@@ -51,12 +81,12 @@ def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0
         while it <= max_iters and not converged:
             R, V = O.assemble(O.K_JACOBIAN, m, model, mats, x, st, quad_order=quad, threads=threads)
             dM = np.zeros(m.num_dofs)
-            for comp, nodes, vals in dbcs(coords, t):
+            for comp, nodes, vals in bcs_of(coords, t):
                 O.dbc(m, nodes, vals, comp, x, R, dM, V)
             du = spla.spsolve(csr(m, V).tocsc(), -R)
             x = x + du
             R, _ = O.assemble(O.K_RESIDUAL, m, model, mats, x, st, quad_order=quad, threads=threads)
-            for comp, nodes, vals in dbcs(coords, t):
+            for comp, nodes, vals in bcs_of(coords, t):
                 O.dbc(m, nodes, vals, comp, x, R)
             norm = np.linalg.norm(R)
             converged = norm < tol
@@ -71,7 +101,7 @@ def newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0
         t_old = t
     out = {"iters": iters, "J": J, "J_steps": Js, "norm": norm}
     if order == 1 and dwr:
-        out.update(nested_estimate(coords, conn, x, st, model, mats, quad, t_old))
+        out.update(nested_estimate(coords, conn, x, st, model, mats, quad, t_old, bcs=bcs_of))
     return out
 
 
@@ -252,18 +282,21 @@ def gpu_known_newton(case, order=1, quad=1, n=2, jitter=0.1, seed=11, lin_rtol=1
 
 
 def gpu_newton_steps(order, quad, model, n, steps, tol=1e-10, max_iters=10, jitter=0.1, seed=11, tangent=64,
-                     solver="device", lin_rtol=1e-12):
+                     solver="device", lin_rtol=1e-12, problem="cube"):
     """The same load steps with the assembly, Dirichlet rows and norms on the
     GPU (through the C ABI); the linear solve is the library's device GMRES
     (solver="device") or scipy's direct solve on the host (solver="scipy")."""
-    disc = g.Disc.box(n, n, n, order=order, jitter=jitter, seed=seed)
+    if problem == "bar":
+        disc = g.Disc.box(*n, *BAR_LENGTHS, order=order, jitter=jitter, seed=seed)
+    else:
+        disc = g.Disc.box(n, n, n, order=order, jitter=jitter, seed=seed)
     mat = dict(zip(("E", "nu", "K", "Y", "c0"), NH_ROW if model == 1 else J2_ROW))
     st = g.States(disc, model, quad)
     mech = g.Mechanics(disc, st, [g.MaterialSpec(model=model, **mat)], stabilization=True, quad_order=quad)
     mech.set_option("tangent", tangent)
     rp, cols = disc.graph()
     si = g.SolInfo(disc)
-    bcs = [(0, "zmin", "0.0"), (1, "zmin", "0.0"), (2, "zmin", "0.0"), (2, "zmax", "0.01*t")]
+    bcs = BAR_BCS if problem == "bar" else CUBE_BCS
     iters, Js, lin = [], [], []
     norm = 0.0
     b, du = g.DeviceArray(disc.num_dofs), g.DeviceArray(disc.num_dofs)

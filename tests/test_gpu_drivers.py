@@ -85,6 +85,25 @@ def test_cube8_p2_newton_matches_oracle():
         assert abs(a - b) <= 1e-9 * abs(b)
 
 
+def test_bar48_j2_tension_matches_oracle():
+    """The J2 tension bar's load path at reduced size: 48 x 6 x 6 hexahedra ->
+    10,368 P1 tets on [0,8]x[0,1]^2, J2 with linear hardening and per-point
+    history, rollers on xmin/ymin/zmin, u_x = 0.004 t on xmax for 20 load
+    steps (1% strain), Newton to ||R|| < 1e-10 with the device GMRES and its
+    multigrid preconditioner.  Same iteration count per load step as the
+    oracle with a direct solve (tests/golden/bar48_j2_newton.json), the
+    average displacement within 1e-9 relative at every step."""
+    ref = json.loads((GOLDEN / "bar48_j2_newton.json").read_text())
+    c = ref["config"]
+    gpu = drivers.gpu_newton_steps(c["order"], c["quad"], c["model"], tuple(c["n"]), c["steps"],
+                                   jitter=c["jitter"], seed=c["seed"], problem=c["problem"])
+    print("newton iterations", gpu["iters"], "linear iterations", gpu["linear_iters"])
+    assert gpu["iters"] == ref["iters"]
+    assert gpu["norm"] < 1e-10
+    for a, b in zip(gpu["J_steps"], ref["J_steps"]):
+        assert abs(a - b) <= 1e-9 * abs(b)
+
+
 @pytest.mark.parametrize("order,quad,model,n,steps", [(1, 1, 1, 3, 3), (2, 2, 1, 2, 2)])
 def test_cpp_surface_with_device_gmres(order, quad, model, n, steps):
     """goal::GmresSolve as the LinearSolve of Primal and NestedAdjoint: the
