@@ -29,23 +29,23 @@ template <int NQ>
 struct Quad;
 template <>
 struct Quad<1> {
-  __device__ static inline void point(int, double xi[3], double& w) {
+  __host__ __device__ static inline void point(int, double xi[3], double& w) {
     xi[0] = xi[1] = xi[2] = 0.25;
     w = 1.0 / 6.0;
   }
   // xi[c] of point q, as point() sets it
-  __device__ static inline double coord(int, int) { return 0.25; }
+  __host__ __device__ static inline double coord(int, int) { return 0.25; }
 };
 template <>
 struct Quad<4> {
-  __device__ static inline void point(int q, double xi[3], double& w) {
+  __host__ __device__ static inline void point(int q, double xi[3], double& w) {
     const double a = 0.5854101966249685, b = 0.1381966011250105;
     xi[0] = q == 1 ? a : b;
     xi[1] = q == 2 ? a : b;
     xi[2] = q == 3 ? a : b;
     w = 1.0 / 24.0;
   }
-  __device__ static inline double coord(int q, int c) {
+  __host__ __device__ static inline double coord(int q, int c) {
     const double a = 0.5854101966249685, b = 0.1381966011250105;
     return q == c + 1 ? a : b;
   }
@@ -53,7 +53,7 @@ struct Quad<4> {
 
 // ------------------------------------------------------------ tet geometry
 // dX/dxi columns X1-X0, X2-X0, X3-X0; grad lambda_{c+1} = row c of its inverse.
-__device__ inline double tet_gradlambda(const double X[4][3], double gl[4][3]) {
+__host__ __device__ inline double tet_gradlambda(const double X[4][3], double gl[4][3]) {
   double A[3][3];
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) A[r][c] = X[c + 1][r] - X[0][r];
@@ -79,7 +79,7 @@ __device__ inline double tet_gradlambda(const double X[4][3], double gl[4][3]) {
 // P1: N = lambda_a.  P2 (apf Lagrange quadratic): vertices lambda(2 lambda-1),
 // edges (i,j) in order 01,12,20,03,13,23: 4 lambda_i lambda_j.
 template <int NN>
-__device__ inline void shape(int a, const double lam[4], const double gl[4][3], double& N, double dN[3]) {
+__host__ __device__ inline void shape(int a, const double lam[4], const double gl[4][3], double& N, double dN[3]) {
   if (NN == 4 || a < 4) {
     if (NN == 4) {
       N = lam[a];
@@ -99,11 +99,11 @@ __device__ inline void shape(int a, const double lam[4], const double gl[4][3], 
 }
 
 // --------------------------------------------------------------- 3x3 tools
-__device__ inline double det3(const double* F) {
+__host__ __device__ inline double det3(const double* F) {
   return F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
          F[2] * (F[3] * F[7] - F[4] * F[6]);
 }
-__device__ inline void inv3(const double* F, double det, double* Fi) {
+__host__ __device__ inline void inv3(const double* F, double det, double* Fi) {
   double id = 1.0 / det;
   Fi[0] = (F[4] * F[8] - F[5] * F[7]) * id;
   Fi[1] = (F[2] * F[7] - F[1] * F[8]) * id;
@@ -117,14 +117,14 @@ __device__ inline void inv3(const double* F, double det, double* Fi) {
 }
 
 // P = J sigma F^{-T}:  P_iJ = J sum_m sigma_im Finv_Jm
-__device__ inline void piola(const double* sig, const double* Fi, double J, double* P) {
+__host__ __device__ inline void piola(const double* sig, const double* Fi, double J, double* P) {
   for (int i = 0; i < 3; ++i)
     for (int c = 0; c < 3; ++c)
       P[3 * i + c] = J * (sig[3 * i] * Fi[3 * c] + sig[3 * i + 1] * Fi[3 * c + 1] + sig[3 * i + 2] * Fi[3 * c + 2]);
 }
 
 // Mixed<T>::at_point: sigma_ii += p - tr(sigma)/3   (src/goal_mixed.cpp:33-46)
-__device__ inline void mixed(double* sig, double p) {
+__host__ __device__ inline void mixed(double* sig, double p) {
   double pbar = (sig[0] + sig[4] + sig[8]) / 3.0;
   sig[0] += p - pbar;
   sig[4] += p - pbar;
@@ -133,7 +133,7 @@ __device__ inline void mixed(double* sig, double p) {
 
 // Matrix exponential by scaling and squaring of a degree-16 Taylor series
 // (used for the plastic deformation update Fp = exp(dgam N) Fp_old).
-__device__ inline void expm3(const double* A, double* E) {
+__host__ __device__ inline void expm3(const double* A, double* E) {
   double nrm = 0.0;
   for (int k = 0; k < 9; ++k) nrm = fmax(nrm, fabs(A[k]));
   int s = 0;
@@ -184,7 +184,7 @@ struct Dir {
 // (src/goal_stabilization.cpp:68-81); Cinv = Finv Finv^T.
 struct Stab {
   double tau, Cinv[9], JCg[3];
-  __device__ inline void init(const QP& q, double tau_) {
+  __host__ __device__ inline void init(const QP& q, double tau_) {
     tau = tau_;
     for (int I = 0; I < 3; ++I)
       for (int K = 0; K < 3; ++K)
@@ -192,12 +192,12 @@ struct Stab {
     for (int j = 0; j < 3; ++j)
       JCg[j] = q.J * (Cinv[j] * q.g[0] + Cinv[3 + j] * q.g[1] + Cinv[6 + j] * q.g[2]);
   }
-  __device__ inline void value(double* s3) const {
+  __host__ __device__ inline void value(double* s3) const {
     for (int j = 0; j < 3; ++j) s3[j] = tau * JCg[j];
   }
   // direction dF = e_k e_L^T:
   //   dJ = J Finv_Lk,  dCinv_IJ = -Finv_Ik Cinv_LJ - Finv_Jk Cinv_IL
-  __device__ inline void dir(const QP& q, int k, int L, double* ds3) const {
+  __host__ __device__ inline void dir(const QP& q, int k, int L, double* ds3) const {
     double dJJ = q.Fi[3 * L + k];  // dJ / J
     for (int j = 0; j < 3; ++j) {
       double acc = 0.0;
@@ -209,14 +209,14 @@ struct Stab {
     }
   }
   // direction d(grad p) = e_I: ds3_j = tau J Cinv_Ij
-  __device__ inline void dir_g(const QP& q, int I, double* ds3) const {
+  __host__ __device__ inline void dir_g(const QP& q, int I, double* ds3) const {
     for (int j = 0; j < 3; ++j) ds3[j] = tau * q.J * Cinv[3 * I + j];
   }
 };
 
 // dP for a direction of F given sigma, dsigma:
 //   dP_iJ = (dJ/J) P_iJ + J (dsigma F^{-T})_iJ - Finv_Jk P_iL
-__device__ inline void dpiola(const QP& q, const double* P, const double* dsig, int k, int L, double* dP) {
+__host__ __device__ inline void dpiola(const QP& q, const double* P, const double* dsig, int k, int L, double* dP) {
   double dJJ = q.Fi[3 * L + k];
   for (int i = 0; i < 3; ++i)
     for (int c = 0; c < 3; ++c) {
@@ -232,7 +232,7 @@ __device__ inline void dpiola(const QP& q, const double* P, const double* dsig, 
 struct NeoHookean {
   double mu, kappa;
   double Jm53, devb[9], sig[9], P[9];
-  __device__ inline void init(const QP& q, const MatParams& m) {
+  __host__ __device__ inline void init(const QP& q, const MatParams& m) {
     mu = m.mu;
     kappa = m.kappa;
     double Jm13 = 1.0 / cbrt(q.J);
@@ -251,10 +251,10 @@ struct NeoHookean {
     mixed(sig, q.p);
     piola(sig, q.Fi, q.J, P);
   }
-  __device__ inline double r1(const QP& q) const { return q.p / kappa - 0.5 * (q.J - 1.0 / q.J); }
+  __host__ __device__ inline double r1(const QP& q) const { return q.p / kappa - 0.5 * (q.J - 1.0 / q.J); }
   // dF = e_k e_L^T:  dJ = J Finv_Lk, d(J^{-5/3}) = -5/3 J^{-5/3} dJ/J,
   // db_ij = d_ik F_jL + F_iL d_jk, d(pJ) = 1/2 kappa (1 + 1/J^2) dJ.
-  __device__ inline void dir(const QP& q, int k, int L, Dir& d) const {
+  __host__ __device__ inline void dir(const QP& q, int k, int L, Dir& d) const {
     double dJJ = q.Fi[3 * L + k];
     double dJ = q.J * dJJ;
     double dJm53 = -(5.0 / 3.0) * Jm53 * dJJ;
@@ -289,7 +289,7 @@ struct J2Model {
   bool plastic;
   int iters;
   double dgam, alpha, sig[9], P[9], s[9];
-  __device__ inline void init(const QP& q, const MatParams& m, const double* Fp_old, double eqps_o) {
+  __host__ __device__ inline void init(const QP& q, const MatParams& m, const double* Fp_old, double eqps_o) {
     mu = m.mu;
     kappa = m.kappa;
     K = m.K;
@@ -337,7 +337,7 @@ struct J2Model {
   // The scalar Newton of src/goal_J2.cpp:98-121 together with the derivative
   // of each iterate along a direction in which |s| and mubar change by
   // dsmag and dmubar (eqps_old, Y, K are fixed).
-  __device__ inline int newton(double dsmag, double dmubar, double& X, double& dX) const {
+  __host__ __device__ inline int newton(double dsmag, double dmubar, double& X, double& dX) const {
     const double sq23 = sqrt(2.0 / 3.0);
     X = 0.0;
     dX = 0.0;
@@ -362,8 +362,8 @@ struct J2Model {
     }
     return iter;
   }
-  __device__ inline bool failed() const { return plastic && iters >= 30; }
-  __device__ inline void new_Fp(const double* Fp_old, double* Fpn) const {
+  __host__ __device__ inline bool failed() const { return plastic && iters >= 30; }
+  __host__ __device__ inline void new_Fp(const double* Fp_old, double* Fpn) const {
     double A[9], E[9];
     for (int k = 0; k < 9; ++k) A[k] = dgam * (str[k] / smag);
     expm3(A, E);
@@ -371,10 +371,10 @@ struct J2Model {
       for (int j = 0; j < 3; ++j)
         Fpn[3 * i + j] = E[3 * i] * Fp_old[j] + E[3 * i + 1] * Fp_old[3 + j] + E[3 * i + 2] * Fp_old[6 + j];
   }
-  __device__ inline double r1(const QP& q) const { return q.p / kappa - 0.5 * (q.J - 1.0 / q.J); }
+  __host__ __device__ inline double r1(const QP& q) const { return q.p / kappa - 0.5 * (q.J - 1.0 / q.J); }
   // dF = e_k e_L^T with bebar = F Cp^{-1} F^T and G = F Cp^{-1}:
   //   d bebar_ij = d_ik G_jL + G_iL d_jk, with G_jL = (bebar F^{-T})_jL
-  __device__ inline void dir(const QP& q, int k, int L, Dir& d) const {
+  __host__ __device__ inline void dir(const QP& q, int k, int L, Dir& d) const {
     double dJJ = q.Fi[3 * L + k];
     double dJ = q.J * dJJ;
     double dJm23 = -(2.0 / 3.0) * Jm23 * dJJ;
@@ -425,7 +425,7 @@ struct J2Model {
 // r1 = p - (2 mu/3 + lambda) tr(H)                         (src/goal_presidual.cpp:61-69)
 struct Elastic {
   double mu, lambda, sig[9], P[9];
-  __device__ inline void init(const QP& q, const MatParams& m) {
+  __host__ __device__ inline void init(const QP& q, const MatParams& m) {
     mu = m.mu;
     lambda = m.lambda;
     double tr = q.H[0] + q.H[4] + q.H[8];
@@ -435,10 +435,10 @@ struct Elastic {
     mixed(sig, q.p);
     for (int k = 0; k < 9; ++k) P[k] = sig[k];
   }
-  __device__ inline double r1(const QP& q) const {
+  __host__ __device__ inline double r1(const QP& q) const {
     return q.p - (2.0 * mu / 3.0 + lambda) * (q.H[0] + q.H[4] + q.H[8]);
   }
-  __device__ inline void dir(const QP&, int k, int L, Dir& d) const {
+  __host__ __device__ inline void dir(const QP&, int k, int L, Dir& d) const {
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
         d.dsig[3 * i + j] = mu * ((i == k && j == L ? 1.0 : 0.0) + (j == k && i == L ? 1.0 : 0.0)) +
@@ -476,7 +476,7 @@ struct NHBaseT {
 using NHBase = NHBaseT<double>;
 
 template <typename S>
-__device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st, NHBaseT<S>& b) {
+__host__ __device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st, NHBaseT<S>& b) {
   double trb = 0.0;
   for (int k = 0; k < 9; ++k) trb += q.F[k] * q.F[k];
   const double c = m.mu * q.J * m.Jm53;
@@ -513,7 +513,7 @@ __device__ inline void nh_base(const QP& q, const NeoHookean& m, const Stab& st,
 // N_b by w dv scales the result.  T = float evaluates the FP32 tangent from the
 // FP64 point state.
 template <typename T, typename S>
-__device__ inline void nh_direction(const NHBaseT<S>& b, int k, T N, const T* G, T* C) {
+__host__ __device__ inline void nh_direction(const NHBaseT<S>& b, int k, T N, const T* G, T* C) {
   const T c = (T)b.c, J = (T)b.J, tJ = (T)(b.tau * b.J);
   if (k < 3) {
     T FG[3], PG[3], CG[3];
@@ -548,7 +548,7 @@ __device__ inline void nh_direction(const NHBaseT<S>& b, int k, T N, const T* G,
 
 // tau = 1/2 c0 h^2 / mu with h^2 the mean squared edge length
 // (src/goal_stabilization.cpp:59-73).
-__device__ inline double stab_tau(const double X[4][3], const MatParams& m) {
+__host__ __device__ inline double stab_tau(const double X[4][3], const MatParams& m) {
   const int ei[6] = {0, 1, 2, 0, 1, 2};
   const int ej[6] = {1, 2, 0, 3, 3, 3};
   double h = 0.0;
