@@ -155,11 +155,23 @@ template <int NN, int NQ, int MODEL, int KIND, typename TE>
 constexpr bool kPrefetch = MODEL == MODEL_NEOHOOKEAN && KIND != ASM_LOCALIZE && Layout<NN, NQ, MODEL>::T >= 128 &&
                            (std::is_same<TE, double>::value || Layout<NN, NQ, MODEL>::MF);
 
+// J2 assemblies other than the localization take their point states from the
+// thread-per-point pass (j2_point_kernel) instead of phases C1 and C2.
+template <int MODEL, int KIND>
+constexpr bool kPointPass = MODEL == MODEL_J2 && KIND != ASM_LOCALIZE;
+template <int NQ, int G>
+struct SJacD {
+  double D[G][NQ][13][13];
+};
+
 template <int NN, int NQ, int MODEL, int KIND, int G, typename TE = double>
 struct Smem : std::conditional<kPrefetch<NN, NQ, MODEL, KIND, TE>, SPrefetch<NN, G>, SNoPrefetch>::type {
+  static constexpr bool JK = KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT;
   using Ext = typename std::conditional<
-      KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT, SJac<NQ, MODEL, G, TE>,
-      typename std::conditional<KIND == ASM_LOCALIZE, SLoc<NN, NQ, G>, SNone>::type>::type;
+      JK && kPointPass<MODEL, KIND>, SJacD<NQ, G>,
+      typename std::conditional<
+          JK, SJac<NQ, MODEL, G, TE>,
+          typename std::conditional<KIND == ASM_LOCALIZE, SLoc<NN, NQ, G>, SNone>::type>::type>::type;
   SBase<NN, NQ, G> b;
   Ext ext;
 };
@@ -390,7 +402,9 @@ __device__ inline void emit_tile_mfma(const double (*sh)[NN][4], const double* w
 }
 
 template <int NN, int NQ, int MODEL, int KIND, typename TE>
-__global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>::W)) void element_kernel(AsmArgs a) {
+__global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
+                             (kPointPass<MODEL, KIND> ? GOAL_ELEM_WAVES : Layout<NN, NQ, MODEL>::W)) void element_kernel(
+    AsmArgs a) {
   using Lay = Layout<NN, NQ, MODEL>;
   using M = typename ModelOf<MODEL>::type;
   constexpr int G = Lay::G, ND = Lay::ND, T = Lay::T;
@@ -544,108 +558,121 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (Layout<NN, NQ, MODEL>:
         }
       }
     }
-    for (int t = tid; t < nv * NQ; t += T) {
-      const int el = t / NQ, q = t % NQ;
-      const int64_t e = e0 + el;
-      const MatParams mp = a.mats[a.elem_set ? a.elem_set[e] : 0];
-      dev::QP qp;
-      for (int k = 0; k < 9; ++k) qp.H[k] = 0.0;
-      qp.p = 0.0;
-      qp.g[0] = qp.g[1] = qp.g[2] = 0.0;
-#pragma unroll 1
-      for (int n = 0; n < NN; ++n) {
-        const double N = s.sh[el][q][n][0];
-        const double pn = s.x[el][4 * n + 3];
-        qp.p += pn * N;
-        for (int j = 0; j < 3; ++j) {
-          const double dNj = s.sh[el][q][n][1 + j];
-          qp.g[j] += pn * dNj;
-          for (int i = 0; i < 3; ++i) qp.H[3 * i + j] += s.x[el][4 * n + i] * dNj;
-        }
+    if constexpr (kPointPass<MODEL, KIND>) {
+      // the point pass left [P, r1, s3] and the tangent, times w dv, per point
+      constexpr int PS = JAC ? kPointTangent : kPointStress;
+      const double* src = a.pt + (e0 * NQ) * PS;
+      for (int t = tid; t < nv * NQ * PS; t += T) {
+        const int pq = t / PS, r = t % PS;
+        const double v = src[t];
+        if (r < 13) s.sig[pq / NQ][pq % NQ][r] = v;
+        else if constexpr (JAC) sm.ext.D[pq / NQ][pq % NQ][(r - 13) / 13][(r - 13) % 13] = v;
       }
-      for (int k = 0; k < 9; ++k) qp.F[k] = qp.H[k] + ((k % 4 == 0) ? 1.0 : 0.0);
-      qp.J = dev::det3(qp.F);
-      dev::inv3(qp.F, qp.J, qp.Fi);
-      dev::Stab st;
-      st.init(qp, a.stabilization ? dev::stab_tau(s.X[el], mp) : 0.0);
-      const double wdv = s.wdv[el][q];
-      const int64_t sidx = e * NQ + q;
-      M m;
-      if constexpr (MODEL == MODEL_J2) {
-        double Fpo[9];
-        for (int k = 0; k < 9; ++k) Fpo[k] = a.Fp_old[k * nqtot + sidx];
-        m.init(qp, mp, Fpo, a.eqps_old[sidx]);
-        if (m.failed()) atomicOr(a.status, 1);
-        if (a.save_state) {
-          a.eqps[sidx] = m.alpha;
-          if (m.plastic) {
-            double Fpn[9];
-            m.new_Fp(Fpo, Fpn);
-            for (int k = 0; k < 9; ++k) a.Fp[k * nqtot + sidx] = Fpn[k];
+    }
+    if constexpr (!kPointPass<MODEL, KIND>) {
+      for (int t = tid; t < nv * NQ; t += T) {
+        const int el = t / NQ, q = t % NQ;
+        const int64_t e = e0 + el;
+        const MatParams mp = a.mats[a.elem_set ? a.elem_set[e] : 0];
+        dev::QP qp;
+        for (int k = 0; k < 9; ++k) qp.H[k] = 0.0;
+        qp.p = 0.0;
+        qp.g[0] = qp.g[1] = qp.g[2] = 0.0;
+  #pragma unroll 1
+        for (int n = 0; n < NN; ++n) {
+          const double N = s.sh[el][q][n][0];
+          const double pn = s.x[el][4 * n + 3];
+          qp.p += pn * N;
+          for (int j = 0; j < 3; ++j) {
+            const double dNj = s.sh[el][q][n][1 + j];
+            qp.g[j] += pn * dNj;
+            for (int i = 0; i < 3; ++i) qp.H[3 * i + j] += s.x[el][4 * n + i] * dNj;
           }
         }
-      } else {
-        m.init(qp, mp);
-      }
-      if (a.save_state)
-        for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
-      for (int k = 0; k < 9; ++k) s.sig[el][q][k] = m.P[k] * wdv;
-      s.sig[el][q][9] = m.r1(qp) * wdv;
-      double s3[3];
-      st.value(s3);
-      for (int j = 0; j < 3; ++j) s.sig[el][q][10 + j] = s3[j] * wdv;
-      if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
-        dev::nh_base(qp, m, st, sm.ext.qs[el][q].b);
-      } else if constexpr (JAC) {
-        auto& D = sm.ext.D[el][q];
-        sm.ext.qs[el][q].qp = qp;
-        sm.ext.qs[el][q].st = st;
-        sm.ext.qs[el][q].m = m;
-        // p column: dsigma/dp = I (Mixed); grad-p columns: stabilization only
-        double dPp[9], dr1p;
-        if constexpr (MODEL == MODEL_ELASTIC) {
-          for (int k = 0; k < 9; ++k) dPp[k] = (k % 4 == 0) ? 1.0 : 0.0;
-          dr1p = 1.0;
+        for (int k = 0; k < 9; ++k) qp.F[k] = qp.H[k] + ((k % 4 == 0) ? 1.0 : 0.0);
+        qp.J = dev::det3(qp.F);
+        dev::inv3(qp.F, qp.J, qp.Fi);
+        dev::Stab st;
+        st.init(qp, a.stabilization ? dev::stab_tau(s.X[el], mp) : 0.0);
+        const double wdv = s.wdv[el][q];
+        const int64_t sidx = e * NQ + q;
+        M m;
+        if constexpr (MODEL == MODEL_J2) {
+          double Fpo[9];
+          for (int k = 0; k < 9; ++k) Fpo[k] = a.Fp_old[k * nqtot + sidx];
+          m.init(qp, mp, Fpo, a.eqps_old[sidx]);
+          if (m.failed()) atomicOr(a.status, 1);
+          if (a.save_state) {
+            a.eqps[sidx] = m.alpha;
+            if (m.plastic) {
+              double Fpn[9];
+              m.new_Fp(Fpo, Fpn);
+              for (int k = 0; k < 9; ++k) a.Fp[k * nqtot + sidx] = Fpn[k];
+            }
+          }
         } else {
-          for (int i = 0; i < 3; ++i)
-            for (int jj = 0; jj < 3; ++jj) dPp[3 * i + jj] = qp.J * qp.Fi[3 * jj + i];
-          dr1p = 1.0 / mp.kappa;
+          m.init(qp, mp);
         }
-        for (int r = 0; r < 9; ++r) D[r][9] = dPp[r] * wdv;
-        D[9][9] = dr1p * wdv;
-        for (int r = 10; r < 13; ++r) D[r][9] = 0.0;
-        for (int I = 0; I < 3; ++I) {
-          double ds3[3];
-          st.dir_g(qp, I, ds3);
-          for (int r = 0; r < 10; ++r) D[r][10 + I] = 0.0;
-          for (int j = 0; j < 3; ++j) D[10 + j][10 + I] = ds3[j] * wdv;
+        if (a.save_state)
+          for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
+        for (int k = 0; k < 9; ++k) s.sig[el][q][k] = m.P[k] * wdv;
+        s.sig[el][q][9] = m.r1(qp) * wdv;
+        double s3[3];
+        st.value(s3);
+        for (int j = 0; j < 3; ++j) s.sig[el][q][10 + j] = s3[j] * wdv;
+        if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
+          dev::nh_base(qp, m, st, sm.ext.qs[el][q].b);
+        } else if constexpr (JAC) {
+          auto& D = sm.ext.D[el][q];
+          sm.ext.qs[el][q].qp = qp;
+          sm.ext.qs[el][q].st = st;
+          sm.ext.qs[el][q].m = m;
+          // p column: dsigma/dp = I (Mixed); grad-p columns: stabilization only
+          double dPp[9], dr1p;
+          if constexpr (MODEL == MODEL_ELASTIC) {
+            for (int k = 0; k < 9; ++k) dPp[k] = (k % 4 == 0) ? 1.0 : 0.0;
+            dr1p = 1.0;
+          } else {
+            for (int i = 0; i < 3; ++i)
+              for (int jj = 0; jj < 3; ++jj) dPp[3 * i + jj] = qp.J * qp.Fi[3 * jj + i];
+            dr1p = 1.0 / mp.kappa;
+          }
+          for (int r = 0; r < 9; ++r) D[r][9] = dPp[r] * wdv;
+          D[9][9] = dr1p * wdv;
+          for (int r = 10; r < 13; ++r) D[r][9] = 0.0;
+          for (int I = 0; I < 3; ++I) {
+            double ds3[3];
+            st.dir_g(qp, I, ds3);
+            for (int r = 0; r < 10; ++r) D[r][10 + I] = 0.0;
+            for (int j = 0; j < 3; ++j) D[10 + j][10 + I] = ds3[j] * wdv;
+          }
         }
-      }
-      if constexpr (LOC) {
-        for (int i = 0; i < 4; ++i) {
+        if constexpr (LOC) {
+          for (int i = 0; i < 4; ++i) {
+            double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
+            for (int n = 0; n < NN; ++n) {
+              const double zn = sm.ext.zd[el][4 * n + i];
+              z += zn * s.sh[el][q][n][0];
+              for (int j = 0; j < 3; ++j) zg[j] += zn * s.sh[el][q][n][1 + j];
+            }
+            sm.ext.zq[el][q][i] = z;
+            for (int j = 0; j < 3; ++j) sm.ext.zgq[el][q][i][j] = zg[j];
+          }
           double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
           for (int n = 0; n < NN; ++n) {
-            const double zn = sm.ext.zd[el][4 * n + i];
+            const double zn = sm.ext.zc[el][n];
             z += zn * s.sh[el][q][n][0];
             for (int j = 0; j < 3; ++j) zg[j] += zn * s.sh[el][q][n][1 + j];
           }
-          sm.ext.zq[el][q][i] = z;
-          for (int j = 0; j < 3; ++j) sm.ext.zgq[el][q][i][j] = zg[j];
+          sm.ext.zcq[el][q][0] = z;
+          for (int j = 0; j < 3; ++j) sm.ext.zcq[el][q][1 + j] = zg[j];
         }
-        double z = 0.0, zg[3] = {0.0, 0.0, 0.0};
-        for (int n = 0; n < NN; ++n) {
-          const double zn = sm.ext.zc[el][n];
-          z += zn * s.sh[el][q][n][0];
-          for (int j = 0; j < 3; ++j) zg[j] += zn * s.sh[el][q][n][1 + j];
-        }
-        sm.ext.zcq[el][q][0] = z;
-        for (int j = 0; j < 3; ++j) sm.ext.zcq[el][q][1 + j] = zg[j];
       }
     }
     __syncthreads();
 
     // ---- C2: tangent columns d sig / d F_kL, one (element, point, direction) per lane
-    if constexpr (JAC && MODEL != MODEL_NEOHOOKEAN) {
+    if constexpr (JAC && MODEL != MODEL_NEOHOOKEAN && !kPointPass<MODEL, KIND>) {
       for (int t = tid; t < nv * NQ * 9; t += T) {
         const int el = t / (NQ * 9), r = t % (NQ * 9);
         const int q = r / 9, c = r % 9;
@@ -874,6 +901,101 @@ __global__ __launch_bounds__(kGatherThreads) void gather_kernel(GatherArgs g) {
   }
 }
 
+// J2 point pass, one thread per quadrature point (e, q): the work of phases
+// C1 and C2 for J2 (interpolation, kinematics, the radial return of
+// src/goal_J2.cpp:71-143 with its history stores, Mixed, the residual terms
+// and, for the Jacobian kinds, the 13 x 13 point tangent), written to pt for
+// the element pass.  The element kernel's phase C1 runs on one lane per point
+// of a group (16 lanes of a workgroup for four-node tets) under the register
+// budget of the return map; here every lane takes a point and no barrier
+// holds it.  The arithmetic is that of phases B, C1 and C2, in the same order.
+template <int NN, int NQ>
+__global__ __launch_bounds__(256) void j2_point_kernel(AsmArgs a, int jac) {
+  const int64_t npts = a.num_elems * NQ, nqtot = npts;
+  const int PS = jac ? kPointTangent : kPointStress;
+  for (int64_t sidx = (int64_t)blockIdx.x * 256 + threadIdx.x; sidx < npts; sidx += (int64_t)gridDim.x * 256) {
+    const int64_t e = sidx / NQ;
+    const int q = (int)(sidx % NQ);
+    const int32_t* ec = a.conn + e * NN;
+    double X[4][3];
+    for (int v = 0; v < 4; ++v)
+      for (int c = 0; c < 3; ++c) X[v][c] = a.coords[3 * (int64_t)ec[v] + c];
+    double gl[4][3];
+    const double det = dev::tet_gradlambda(X, gl);
+    double xi[3], w;
+    Quad<NQ>::point(q, xi, w);
+    const double lam[4] = {1.0 - xi[0] - xi[1] - xi[2], xi[0], xi[1], xi[2]};
+    const MatParams mp = a.mats[a.elem_set ? a.elem_set[e] : 0];
+    dev::QP qp;
+    for (int k = 0; k < 9; ++k) qp.H[k] = 0.0;
+    qp.p = 0.0;
+    qp.g[0] = qp.g[1] = qp.g[2] = 0.0;
+#pragma unroll 1
+    for (int n = 0; n < NN; ++n) {
+      double N, dN[3];
+      dev::shape<NN>(n, lam, gl, N, dN);
+      const double* xn = a.x + 4 * (int64_t)ec[n];
+      const double pn = xn[3];
+      qp.p += pn * N;
+      for (int j = 0; j < 3; ++j) {
+        qp.g[j] += pn * dN[j];
+        for (int i = 0; i < 3; ++i) qp.H[3 * i + j] += xn[i] * dN[j];
+      }
+    }
+    for (int k = 0; k < 9; ++k) qp.F[k] = qp.H[k] + ((k % 4 == 0) ? 1.0 : 0.0);
+    qp.J = dev::det3(qp.F);
+    dev::inv3(qp.F, qp.J, qp.Fi);
+    dev::Stab st;
+    st.init(qp, a.stabilization ? dev::stab_tau(X, mp) : 0.0);
+    const double wdv = w * det;
+    double Fpo[9];
+    for (int k = 0; k < 9; ++k) Fpo[k] = a.Fp_old[k * nqtot + sidx];
+    dev::J2Model m;
+    m.init(qp, mp, Fpo, a.eqps_old[sidx]);
+    if (m.failed()) atomicOr(a.status, 1);
+    if (a.save_state) {
+      a.eqps[sidx] = m.alpha;
+      if (m.plastic) {
+        double Fpn[9];
+        m.new_Fp(Fpo, Fpn);
+        for (int k = 0; k < 9; ++k) a.Fp[k * nqtot + sidx] = Fpn[k];
+      }
+      for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
+    }
+    double* o = a.pt + sidx * PS;
+    for (int k = 0; k < 9; ++k) o[k] = m.P[k] * wdv;
+    o[9] = m.r1(qp) * wdv;
+    double s3[3];
+    st.value(s3);
+    for (int j = 0; j < 3; ++j) o[10 + j] = s3[j] * wdv;
+    if (!jac) continue;
+    double* D = o + 13;  // D[r][c] at D[13 r + c]
+    // p column: dsigma/dp = I (Mixed); grad-p columns: stabilization only
+    for (int i = 0; i < 3; ++i)
+      for (int jj = 0; jj < 3; ++jj) D[13 * (3 * i + jj) + 9] = (qp.J * qp.Fi[3 * jj + i]) * wdv;
+    D[13 * 9 + 9] = (1.0 / mp.kappa) * wdv;
+    for (int r = 10; r < 13; ++r) D[13 * r + 9] = 0.0;
+    for (int I = 0; I < 3; ++I) {
+      double ds3[3];
+      st.dir_g(qp, I, ds3);
+      for (int r = 0; r < 10; ++r) D[13 * r + 10 + I] = 0.0;
+      for (int j = 0; j < 3; ++j) D[13 * (10 + j) + 10 + I] = ds3[j] * wdv;
+    }
+    // F columns: directional derivatives through the return map (unrolled:
+    // the direction indexes the point's 3x3 arrays, which registers cannot)
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      dev::Dir d;
+      m.dir(qp, c / 3, c % 3, d);
+      double ds3[3];
+      st.dir(qp, c / 3, c % 3, ds3);
+      for (int rr = 0; rr < 9; ++rr) D[13 * rr + c] = d.dP[rr] * wdv;
+      D[13 * 9 + c] = d.dr1 * wdv;
+      for (int j = 0; j < 3; ++j) D[13 * (10 + j) + c] = ds3[j] * wdv;
+    }
+  }
+}
+
 template <int NN, int NQ, int MODEL, int KIND, typename TE>
 int occupancy() {
   static const int occ = [] {
@@ -904,6 +1026,13 @@ int resident_grid(int64_t ne) {
 
 template <int NN, int NQ, int MODEL, int KIND, typename TE>
 hipError_t launch4(const AsmArgs& a, int blocks, hipStream_t s) {
+  if constexpr (kPointPass<MODEL, KIND>) {
+    if (!a.pt) return hipErrorInvalidValue;
+    const int64_t npts = a.num_elems * NQ;
+    const int pb = (int)std::min<int64_t>((npts + 255) / 256, 65536);
+    hipLaunchKernelGGL((j2_point_kernel<NN, NQ>), dim3(pb), dim3(256), 0, s, a,
+                       (KIND == ASM_JACOBIAN || KIND == ASM_ADJOINT) ? 1 : 0);
+  }
   if (blocks <= 0) blocks = resident_grid<NN, NQ, MODEL, KIND, TE>(a.num_elems);
   hipLaunchKernelGGL((element_kernel<NN, NQ, MODEL, KIND, TE>), dim3(blocks), dim3(Layout<NN, NQ, MODEL>::T), 0, s,
                      a);
@@ -1089,6 +1218,9 @@ int group_size_of(int model) {
   }
 }
 }  // namespace
+
+int point_stride(int kind) { return (kind == ASM_JACOBIAN || kind == ASM_ADJOINT) ? kPointTangent : kPointStress; }
+bool uses_point_pass(int model, int kind) { return model == MODEL_J2 && kind != ASM_LOCALIZE; }
 
 int element_group_size(int order, int quad_order, int model) {
   const int nq = quad_order == 1 ? 1 : 4;

@@ -279,6 +279,7 @@ struct goal_mech_s {
   DevBuf<int32_t> status;
   DevBuf<double> elem_value;
   DevBuf<double> emr, em;  // element residuals / matrices between the two passes
+  DevBuf<double> pt;       // J2 point states between the point pass and the element pass
   DevBuf<double> feat_vol, feat_vm;  // learned-indicator feature scratch
   bool timing = false, timed = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -545,19 +546,30 @@ void run_chunked(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool wa
 
 // Element buffers between the two passes: the mechanics' own (allocated on
 // first use), or, with emr_buf/em_buf given, the caller's.
-// With emr_buf given the assembly also stays in one pass (the chunked path
-// keeps a host-side table it builds on first use); the two give the same bits.
+// Doubles of J2 point states an assembly of this kind needs (0: none).
+int64_t point_states(goal_mech m, int kind) {
+  if (!goal::uses_point_pass(m->model, kind)) return 0;
+  return m->d->mesh.num_elems * m->st->nq * goal::point_stride(kind);
+}
+
+// With emr_buf given (and pt_buf for J2) the assembly also stays in one pass
+// (the chunked path keeps a host-side table it builds on first use); the two
+// give the same bits.
 void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool want_vals, double* emr_buf = nullptr,
-                  double* em_buf = nullptr) {
+                  double* em_buf = nullptr, double* pt_buf = nullptr) {
   const bool own_buffers = emr_buf == nullptr;
   goal_disc d = m->d;
   const int64_t nd = 4 * (int64_t)d->mesh.nodes_per_elem;
   if (!emr_buf) {
     if (m->emr.n < d->mesh.num_elems * nd) m->emr.alloc(d->mesh.num_elems * nd);
     if (want_vals && m->em.n < d->mesh.num_elems * nd * nd) m->em.alloc(d->mesh.num_elems * nd * nd);
+    const int64_t np = point_states(m, kind);
+    if (m->pt.n < np) m->pt.alloc(np);
     emr_buf = m->emr.p;
     em_buf = m->em.p;
+    pt_buf = m->pt.p;
   }
+  a.pt = pt_buf;
   a.rec = d->rec.p;
   a.emr = emr_buf;
   const bool f32 = want_vals && m->tangent == 32;
@@ -1879,20 +1891,24 @@ int goal_asm_jacobian_noalloc(goal_mech m, goal_solinfo s, int adjoint, int save
     require(m && s && s->d == m->d, "goal_asm_jacobian_noalloc: bad arguments");
     goal::AsmArgs a = asm_args(m);  // ctx() is up: the mechanics was created through it
     const int64_t ne = m->d->mesh.num_elems, nd = 4 * (int64_t)m->d->mesh.nodes_per_elem;
-    // element matrices (FP64; FP32 ones take half) first, then element residuals
-    const int64_t need = ne * nd * nd + ne * nd;
+    // element matrices (FP64; FP32 ones take half), element residuals, then
+    // (J2) the point states
+    const int kind = adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN;
+    const int64_t per_elem = nd * nd + nd + (point_states(m, kind) > 0 ? m->st->nq * goal::point_stride(kind) : 0);
+    const int64_t need = ne * per_elem;
     if (need > kNoallocScratchDoubles)
       return set_error(GOAL_ERR_INVALID,
                        "goal_asm_jacobian_noalloc: %lld elements need %lld doubles of scratch, more than its "
-                       "%lld (at most %lld %s tets)",
+                       "%lld (at most %lld elements of this mesh and model)",
                        (long long)ne, (long long)need, (long long)kNoallocScratchDoubles,
-                       (long long)(kNoallocScratchDoubles / (nd * nd + nd)), nd == 40 ? "ten-node" : "four-node");
+                       (long long)(kNoallocScratchDoubles / per_elem));
     double* base = g_noalloc_ptr[ctx().device];
     require(base != nullptr, "goal_asm_jacobian_noalloc: scratch address not set up for this device");
     a.save_state = save_states ? 1 : 0;
-    // every value the gather reads is written by this call's element pass, so
-    // nothing left from an earlier call needs clearing
-    run_assembly(m, s, adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN, a, true, base + ne * nd * nd, base);
+    // every value the gather reads is written by this call's element pass, and
+    // every point state the element pass reads by its point pass, so nothing
+    // left from an earlier call needs clearing
+    run_assembly(m, s, kind, a, true, base + ne * nd * nd, base, base + ne * (nd * nd + nd));
     return finish_assembly(m);
   });
 }

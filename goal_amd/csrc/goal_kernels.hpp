@@ -59,7 +59,18 @@ struct AsmArgs {
   // neo-Hookean only: this launch takes element groups [group_lo, group_hi)
   // (group_hi == 0: all of them), so an assembly can be split into chunks
   int64_t group_lo, group_hi;
+  // J2 (all kinds but ASM_LOCALIZE): the point states, written by a
+  // thread-per-point pass before the element pass reads them.  Per point
+  // (e * nq + q): the generalized stress [P, r1, s3] (13) and, for the Jacobian
+  // kinds, the point tangent D (13 x 13, row-major), all times w dv.
+  double* pt;
 };
+
+// Doubles per point in AsmArgs::pt: residual kinds, Jacobian kinds.
+constexpr int kPointStress = 13;
+constexpr int kPointTangent = 13 + 169;
+int point_stride(int kind);
+bool uses_point_pass(int model, int kind);
 
 // Node-owner gather of element residuals / matrices into R and the CSR values.
 struct GatherArgs {
