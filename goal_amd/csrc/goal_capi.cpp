@@ -623,12 +623,16 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
 //     matrices the Galerkin sums P^T A P, down to at most kAmgCoarseNodes nodes,
 //     whose matrix is inverted densely.
 //   - Smoother: Chebyshev of degree kAmgDegree on Binv A (Binv: inverse 4x4
-//     diagonal blocks), on [lmax / 30, 1.1 lmax] with lmax from a power
-//     iteration, before and after the coarse correction.
+//     diagonal blocks), on [lmax / 10, 1.1 lmax] with lmax from a power
+//     iteration, before and after the coarse correction.  (A numpy model of
+//     this hierarchy on the P2 cube's and the plastic J2 bar's Jacobians:
+//     degree 3 on [lmax/30, ..] took 63 and 132-178 GMRES iterations to 1e-8,
+//     degree 4 on [lmax/10, ..] 57 and 69-92; block Jacobi alone 800+ and
+//     did not converge in 1,500-2,000.)
 // Every kernel sums in a fixed order, so a solve is bitwise reproducible.
 constexpr int kAmgCoarseNodes = 128;
 constexpr int kAmgMaxLevels = 12;
-constexpr int kAmgDegree = 3;
+constexpr int kAmgDegree = 4;
 constexpr int kAmgPowerIters = 12;
 
 void amg_build_structure(goal_disc d) {
@@ -769,7 +773,7 @@ void amg_setup(goal_solinfo s, int* status) {
 // x = Chebyshev smoothing of A_l x = b (from x = 0 when zero).
 void amg_smooth(goal_solinfo s, AmgHierarchy& h, int l, const double* b, double* x, bool zero) {
   AmgLevel& L = *h.lv[l];
-  const double hi = 1.1 * L.lmax, lo = hi / 30.0;
+  const double hi = 1.1 * L.lmax, lo = hi / 10.0;
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
   double rho = 1.0 / sigma;
   const double* r = b;

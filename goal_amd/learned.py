@@ -118,8 +118,9 @@ class Model:
         self.ne = self.conn.shape[0]
         self.newton_info = {}
 
-    def newton(self, t=1.0, tol=1e-10, max_iters=12, lin_rtol=1e-8, lin_max=20000, restart=100) -> dict:
-        """Primal::solve (src/goal_primal.cpp:111-137) with the device GMRES."""
+    def newton(self, t=1.0, tol=1e-10, max_iters=12, lin_rtol=1e-8, lin_max=20000, restart=100, log=None) -> dict:
+        """Primal::solve (src/goal_primal.cpp:111-137) with the device GMRES;
+        log, if given, is called after every iteration."""
         n = self.disc.num_dofs
         b, du = g.DeviceArray(n), g.DeviceArray(n)
         lin, it, norm = [], 0, np.inf
@@ -141,6 +142,8 @@ class Model:
                 g.dbc_apply(self.disc, self.si, comp, ns, expr, t, False)
             norm = self.si.norm_R()
             it += 1
+            if log:
+                log(f"  newton {it}: {k} GMRES iterations, ||R|| = {norm:.3e} ({time.perf_counter() - t0:.1f} s)")
             if norm < tol:
                 break
         g.sync()
@@ -442,7 +445,7 @@ def run_cycles(p: Problem, method: str, cycles: int, final_elems: int, net: Lear
     growth = (final_elems / ne0) ** (1.0 / cycles)
     out = []
     for c in range(cycles + 1):
-        info = m.newton()
+        info = m.newton(log=log)
         J = m.qoi()
         rec = {"cycle": c, "elems": m.ne, "J": J, "newton": info["iters"], "converged": info["converged"],
                "newton_seconds": round(info["seconds"], 4), "linear_iters": info["linear_iters"]}
@@ -489,7 +492,7 @@ def overkill_qoi(m: Model, log=print) -> dict:
     del ep
     ref = Model(m.p, fc, fconn)
     ref.disc.set_soln(xf.ravel())
-    info = ref.newton()
+    info = ref.newton(log=log)
     J = ref.qoi()
     if log:
         log(f"[learned] overkill: {ref.ne} tets, J {J:.10g}, newton {info['iters']} ({info['seconds']:.1f} s), "
