@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 B_ELEM_P2 = 13576  # algorithmic bytes per P2/P2 element, residual + Jacobian pass
 B_ELEM_P1 = 2416
 NCOPY = 1 << 28  # doubles per buffer of the in-run copy kernel (2 GiB read + 2 GiB written)
-ASSEMBLY_KERNELS = ("element_kernel", "gather_kernel")
+ASSEMBLY_KERNELS = ("j2_point_kernel", "element_kernel", "gather_kernel")  # the point pass runs for J2 only
 PROFILES = os.path.join(ROOT, "profiles")
 
 
@@ -340,7 +340,8 @@ def summarize(kt_dir, fetch_dir, write_dir, tag, commands, tangent=64):
             kern[k] = {"calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"])}
     out = {"tag": tag, "commands": commands, "calibration": {"fetch": cal_fetch, "write": cal_write},
            "kernels": {}}
-    for k in ASSEMBLY_KERNELS:
+    present = [k for k in ASSEMBLY_KERNELS if k in kern]
+    for k in present:
         fk, wk = mean[k]["FETCH_SIZE"], mean[k]["WRITE_SIZE"]
         out["kernels"][k] = {"avg_ms": kern[k]["avg_ns"] / 1e6, "fetch_kib": fk, "write_kib": wk,
                              "read_bytes": fk * 1024.0 * cal_fetch, "write_bytes": wk * 1024.0 * cal_write}
@@ -364,15 +365,96 @@ def summarize(kt_dir, fetch_dir, write_dir, tag, commands, tangent=64):
               f"{cal_fetch:.4f} and WRITE_SIZE by {cal_write:.4f}.", "",
               "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB (scaled) | written GB (scaled) |",
               "|---|---|---|---|---|"]
-    for k in ASSEMBLY_KERNELS + ("copy_kernel",):
+    for k in present + ["copy_kernel"]:
         fk, wk = mean[k]["FETCH_SIZE"], mean[k]["WRITE_SIZE"]
         lines.append(f"| {k} | {fk:.0f} | {wk:.0f} | {fk * 1024 * cal_fetch / 1e9:.3f} | "
                      f"{wk * 1024 * cal_write / 1e9:.3f} |")
-    lines += ["", f"HBM traffic per assembly (element + gather): {out['traffic_bytes'] / 1e9:.3f} GB", ""]
+    lines += ["", f"HBM traffic per assembly ({' + '.join(present)}): {out['traffic_bytes'] / 1e9:.3f} GB", ""]
     with open(os.path.join(PROFILES, f"{tag}.md"), "w") as f:
         f.write("\n".join(lines))
     print(json.dumps({"tag": tag, "traffic_GB": out["traffic_bytes"] / 1e9,
                       "avg_ms": {k: v["avg_ms"] for k, v in out["kernels"].items()}}))
+    return 0
+
+
+def summarize_batch(kt_dir, fetch_dir, write_dir, commands, replays):
+    """profiles/assembly_batch64.md and its _traffic.json from rocprofv3 CSV
+    output of `bench.py --workload batch`: the last `replays` graph replays of
+    the kernel trace (each one dispatch per kernel node of the graph), with
+    HBM traffic from the FETCH_SIZE / WRITE_SIZE runs over the same dispatches."""
+    specs = batch_meshes()
+    nodes = 3 * len(specs) + sum(1 for sp_ in specs if sp_["model"] == 2)  # fill, element, gather; J2 point pass
+    trace = sorted(_read_csv(_find(kt_dir, "kernel_trace.csv")), key=lambda r: int(r["Start_Timestamp"]))
+    timed = trace[-replays * nodes:]
+    spans = []
+    for k in range(replays):
+        rr = timed[k * nodes:(k + 1) * nodes]
+        spans.append((max(int(r["End_Timestamp"]) for r in rr) - min(int(r["Start_Timestamp"]) for r in rr)) / 1e6)
+
+    def short(name):
+        base = name.replace("void ", "").replace("goal::(anonymous namespace)::", "")
+        return base.split("(")[0]
+
+    per = {}
+    for r in timed:
+        k = short(r["Kernel_Name"])
+        d = per.setdefault(k, {"n": 0, "ns": 0.0})
+        d["n"] += 1
+        d["ns"] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    # counters: the last replays' dispatches of each counter run, per kernel name
+    counters = {}
+    for which, dd in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = [r for r in _read_csv(_find(dd, "counter_collection.csv")) if r["Counter_Name"] == which]
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        for r in rows[-replays * nodes:]:
+            k = short(r["Kernel_Name"])
+            counters.setdefault(k, {}).setdefault(which, 0.0)
+            counters[k][which] += float(r["Counter_Value"])
+    cal = {"fetch": 2.0, "write": 1.0}  # the copy-kernel calibration of the cube profiles
+    kernels = {}
+    for k, d in per.items():
+        c = counters.get(k, {})
+        kernels[k] = {"dispatches_per_replay": d["n"] / replays, "ms_per_replay": d["ns"] / 1e6 / replays,
+                      "read_bytes": c.get("FETCH_SIZE", 0.0) * 1024 * cal["fetch"] / replays,
+                      "write_bytes": c.get("WRITE_SIZE", 0.0) * 1024 * cal["write"] / replays}
+    traffic = sum(v["read_bytes"] + v["write_bytes"] for v in kernels.values())
+    ne = [6 * sp_["dims"][0] * sp_["dims"][1] * sp_["dims"][2] for sp_ in specs]
+    alg = sum((B_ELEM_P2 if sp_["order"] == 2 else B_ELEM_P1 + B_STATE_J2) * n for sp_, n in zip(specs, ne))
+    out = {"tag": "assembly_batch64", "commands": commands, "calibration": cal,
+           "replay_ms": statistics.mean(spans), "kernel_ms_per_replay": sum(v["ms_per_replay"] for v in kernels.values()),
+           "kernels": kernels, "traffic_bytes": traffic, "algorithmic_bytes": alg}
+    with open(os.path.join(PROFILES, "assembly_batch64_traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    stats = _read_csv(_find(kt_dir, "kernel_stats.csv"))
+    lines = ["# assembly_batch64: kernel trace and HBM traffic of one graph replay", "",
+             f"The batched workload: {len(specs)} seeded box meshes, {sum(ne):,} tets in all "
+             f"({min(ne):,} to {max(ne):,} per mesh). Even meshes are P2 neo-Hookean with 4 points, odd ones P1 J2 "
+             f"with 1 point and per-point history.",
+             f"One captured HIP graph holds every mesh's residual + Jacobian assembly: per mesh one fill, one element "
+             f"kernel and one gather, and for the J2 meshes the point pass before the element kernel: {nodes} "
+             f"dispatches per replay.", "", "Commands (repository root on the GPU host):", ""]
+    lines += [f"    {c}" for c in commands]
+    lines += ["", "## rocprofv3 --kernel-trace --stats (whole run: autotuning, capture, warm-up and timed replays)",
+              "", "| kernel | calls | average ms | total ms | % |", "|---|---|---|---|---|"]
+    for r in stats:
+        lines.append(f"| `{r['Name'][:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} |")
+    lines += ["", f"## The {replays} timed replays (the last {replays * nodes} dispatches of the trace)", "",
+              "Replay spans, first dispatch start to last dispatch end: "
+              + ", ".join(f"{x:.3f}" for x in spans) + " ms.", "",
+              "Traffic is from the two counter runs over the same dispatches, FETCH_SIZE ×2 and WRITE_SIZE ×1 "
+              "(the copy-kernel calibration of `assembly_p2_nh_cube64.md`).", "",
+              "| kernel | dispatches per replay | ms per replay | read GB per replay | written GB per replay |",
+              "|---|---|---|---|---|"]
+    for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_replay"]):
+        lines.append(f"| `{k}` | {v['dispatches_per_replay']:.0f} | {v['ms_per_replay']:.3f} | "
+                     f"{v['read_bytes'] / 1e9:.3f} | {v['write_bytes'] / 1e9:.3f} |")
+    lines += ["", f"Per replay: {out['kernel_ms_per_replay']:.3f} ms of kernel time and {traffic / 1e9:.2f} GB of HBM "
+              f"traffic; algorithmic bytes {alg / 1e9:.2f} GB (13,576 B per P2 tet, 2,648 B per P1 J2 tet), "
+              f"ratio {traffic / alg:.2f}.", ""]
+    with open(os.path.join(PROFILES, "assembly_batch64.md"), "w") as f:
+        f.write("\n".join(lines))
+    print(json.dumps({"tag": "assembly_batch64", "replay_ms": out["replay_ms"], "traffic_GB": traffic / 1e9}))
     return 0
 
 
@@ -384,7 +466,9 @@ def load_traffic(tag, element_ms, gather_ms):
         return None, "no profile"
     with open(path) as f:
         prof = json.load(f)
-    ke, kg = prof["kernels"]["element_kernel"]["avg_ms"], prof["kernels"]["gather_kernel"]["avg_ms"]
+    # the library's element-pass events bracket the point pass too
+    ke = sum(prof["kernels"][k]["avg_ms"] for k in ("j2_point_kernel", "element_kernel") if k in prof["kernels"])
+    kg = prof["kernels"]["gather_kernel"]["avg_ms"]
     if abs(ke - element_ms) > 0.15 * element_ms or abs(kg - gather_ms) > 0.15 * gather_ms:
         return None, f"stale profile (element {ke:.2f} ms, gather {kg:.2f} ms)"
     return prof["traffic_bytes"], os.path.relpath(path, ROOT)
@@ -758,6 +842,12 @@ def main() -> int:
                     help="write profiles/ summaries from rocprofv3 CSV output of this benchmark and exit")
     args = ap.parse_args()
 
+    if args.summarize and args.workload == "batch":
+        base = f"python3 bench.py --workload batch --steps {args.steps} --warmup 1"
+        cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d {args.summarize[0]} -o run -- {base}",
+                f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {args.summarize[1]} -o run -- {base}",
+                f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {args.summarize[2]} -o run -- {base}"]
+        return summarize_batch(*args.summarize, cmds, args.steps)
     if args.summarize:
         base = (f"python3 bench.py --workload {args.workload} --iterate synthetic"
                 + (f" --overlap {args.overlap}" if args.overlap >= 0 else "")

@@ -423,6 +423,16 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
   }
   const int64_t nqtot = ne * NQ;
   bool fetched = false;  // this group's phase-A data is in the prefetch buffers
+  if constexpr (kPointPass<MODEL, KIND> && JAC) {
+    // the point tangent's structural zeros (s3 rows of the p column, P and r1
+    // rows of the grad-p columns): never stored, never overwritten
+    for (int t = tid; t < G * NQ * 33; t += T) {
+      auto& D = sm.ext.D[t / (NQ * 33)][(t / 33) % NQ];
+      const int z = t % 33;
+      if (z < 3) D[10 + z][9] = 0.0;
+      else D[(z - 3) % 10][10 + (z - 3) / 10] = 0.0;
+    }
+  }
 
   for (int64_t it = 0;; ++it) {
     const int64_t grp0 = group_of(it, ngroups, a.xcd_remap);
@@ -453,12 +463,13 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
       const int el = t / 12, c = t % 12;
       s.X[el][c / 3][c % 3] = a.coords[3 * (int64_t)s.conn[el][c / 3] + (c % 3)];
     }
-    for (int t = tid; t < nv * ND; t += T) {
-      const int el = t / ND, k = t % ND;
-      const int64_t dof = 4 * (int64_t)s.conn[el][k >> 2] + (k & 3);
-      s.x[el][k] = a.x[dof];
-      if constexpr (LOC) sm.ext.zd[el][k] = a.z_diff[dof];
-    }
+    if constexpr (!kPointPass<MODEL, KIND>)  // the point pass read the DOFs
+      for (int t = tid; t < nv * ND; t += T) {
+        const int el = t / ND, k = t % ND;
+        const int64_t dof = 4 * (int64_t)s.conn[el][k >> 2] + (k & 3);
+        s.x[el][k] = a.x[dof];
+        if constexpr (LOC) sm.ext.zd[el][k] = a.z_diff[dof];
+      }
     if constexpr (LOC)
       for (int t = tid; t < nv * NN; t += T) sm.ext.zc[t / NN][t % NN] = a.z_coarse[4 * (int64_t)s.conn[t / NN][t % NN] + 3];
     }
@@ -561,12 +572,18 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
     if constexpr (kPointPass<MODEL, KIND>) {
       // the point pass left [P, r1, s3] and the tangent, times w dv, per point
       constexpr int PS = JAC ? kPointTangent : kPointStress;
-      const double* src = a.pt + (e0 * NQ) * PS;
-      for (int t = tid; t < nv * NQ * PS; t += T) {
-        const int pq = t / PS, r = t % PS;
-        const double v = src[t];
-        if (r < 13) s.sig[pq / NQ][pq % NQ][r] = v;
-        else if constexpr (JAC) sm.ext.D[pq / NQ][pq % NQ][(r - 13) / 13][(r - 13) % 13] = v;
+      const int npq = nv * NQ;
+      for (int t = tid; t < npq * PS; t += T) {
+        const int r = t / npq, pq = t % npq;  // consecutive lanes: consecutive points
+        const double v = a.pt[point_state_index(e0 * NQ + pq, r, PS)];
+        if (r < 13) {
+          s.sig[pq / NQ][pq % NQ][r] = v;
+        } else if constexpr (JAC) {
+          auto& D = sm.ext.D[pq / NQ][pq % NQ];
+          if (r < 130) D[(r - 13) % 13][(r - 13) / 13] = v;
+          else if (r < 140) D[r - 130][9] = v;
+          else D[10 + (r - 140) % 3][10 + (r - 140) / 3] = v;
+        }
       }
     }
     if constexpr (!kPointPass<MODEL, KIND>) {
@@ -962,24 +979,23 @@ __global__ __launch_bounds__(256) void j2_point_kernel(AsmArgs a, int jac) {
       }
       for (int k = 0; k < 9; ++k) a.sigma[k * nqtot + sidx] = m.sig[k];
     }
-    double* o = a.pt + sidx * PS;
-    for (int k = 0; k < 9; ++k) o[k] = m.P[k] * wdv;
-    o[9] = m.r1(qp) * wdv;
+    // value k at o[64 k]: a wave's 64 consecutive points store each value as
+    // one contiguous run
+    double* o = a.pt + point_state_index(sidx, 0, PS);
+    for (int k = 0; k < 9; ++k) o[64 * k] = m.P[k] * wdv;
+    o[64 * 9] = m.r1(qp) * wdv;
     double s3[3];
     st.value(s3);
-    for (int j = 0; j < 3; ++j) o[10 + j] = s3[j] * wdv;
+    for (int j = 0; j < 3; ++j) o[64 * (10 + j)] = s3[j] * wdv;
     if (!jac) continue;
-    double* D = o + 13;  // D[r][c] at D[13 r + c]
     // p column: dsigma/dp = I (Mixed); grad-p columns: stabilization only
     for (int i = 0; i < 3; ++i)
-      for (int jj = 0; jj < 3; ++jj) D[13 * (3 * i + jj) + 9] = (qp.J * qp.Fi[3 * jj + i]) * wdv;
-    D[13 * 9 + 9] = (1.0 / mp.kappa) * wdv;
-    for (int r = 10; r < 13; ++r) D[13 * r + 9] = 0.0;
+      for (int jj = 0; jj < 3; ++jj) o[64 * (130 + 3 * i + jj)] = (qp.J * qp.Fi[3 * jj + i]) * wdv;
+    o[64 * 139] = (1.0 / mp.kappa) * wdv;
     for (int I = 0; I < 3; ++I) {
       double ds3[3];
       st.dir_g(qp, I, ds3);
-      for (int r = 0; r < 10; ++r) D[13 * r + 10 + I] = 0.0;
-      for (int j = 0; j < 3; ++j) D[13 * (10 + j) + 10 + I] = ds3[j] * wdv;
+      for (int j = 0; j < 3; ++j) o[64 * (140 + 3 * I + j)] = ds3[j] * wdv;
     }
     // F columns: directional derivatives through the return map (unrolled:
     // the direction indexes the point's 3x3 arrays, which registers cannot)
@@ -989,9 +1005,10 @@ __global__ __launch_bounds__(256) void j2_point_kernel(AsmArgs a, int jac) {
       m.dir(qp, c / 3, c % 3, d);
       double ds3[3];
       st.dir(qp, c / 3, c % 3, ds3);
-      for (int rr = 0; rr < 9; ++rr) D[13 * rr + c] = d.dP[rr] * wdv;
-      D[13 * 9 + c] = d.dr1 * wdv;
-      for (int j = 0; j < 3; ++j) D[13 * (10 + j) + c] = ds3[j] * wdv;
+      double* oc = o + 64 * (13 + 13 * c);
+      for (int rr = 0; rr < 9; ++rr) oc[64 * rr] = d.dP[rr] * wdv;
+      oc[64 * 9] = d.dr1 * wdv;
+      for (int j = 0; j < 3; ++j) oc[64 * (10 + j)] = ds3[j] * wdv;
     }
   }
 }

@@ -549,7 +549,7 @@ void run_chunked(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool wa
 // Doubles of J2 point states an assembly of this kind needs (0: none).
 int64_t point_states(goal_mech m, int kind) {
   if (!goal::uses_point_pass(m->model, kind)) return 0;
-  return m->d->mesh.num_elems * m->st->nq * goal::point_stride(kind);
+  return goal::point_state_doubles(m->d->mesh.num_elems * m->st->nq, goal::point_stride(kind));
 }
 
 // With emr_buf given (and pt_buf for J2) the assembly also stays in one pass
@@ -1898,14 +1898,19 @@ int goal_asm_jacobian_noalloc(goal_mech m, goal_solinfo s, int adjoint, int save
     // element matrices (FP64; FP32 ones take half), element residuals, then
     // (J2) the point states
     const int kind = adjoint ? goal::ASM_ADJOINT : goal::ASM_JACOBIAN;
-    const int64_t per_elem = nd * nd + nd + (point_states(m, kind) > 0 ? m->st->nq * goal::point_stride(kind) : 0);
-    const int64_t need = ne * per_elem;
-    if (need > kNoallocScratchDoubles)
+    const bool pts = goal::uses_point_pass(m->model, kind);
+    auto need_of = [&](int64_t n) {
+      return n * (nd * nd + nd) + (pts ? goal::point_state_doubles(n * m->st->nq, goal::point_stride(kind)) : 0);
+    };
+    const int64_t need = need_of(ne);
+    if (need > kNoallocScratchDoubles) {
+      int64_t most = kNoallocScratchDoubles / (nd * nd + nd + (pts ? m->st->nq * goal::point_stride(kind) : 0));
+      while (most > 0 && need_of(most) > kNoallocScratchDoubles) --most;
       return set_error(GOAL_ERR_INVALID,
                        "goal_asm_jacobian_noalloc: %lld elements need %lld doubles of scratch, more than its "
                        "%lld (at most %lld elements of this mesh and model)",
-                       (long long)ne, (long long)need, (long long)kNoallocScratchDoubles,
-                       (long long)(kNoallocScratchDoubles / per_elem));
+                       (long long)ne, (long long)need, (long long)kNoallocScratchDoubles, (long long)most);
+    }
     double* base = g_noalloc_ptr[ctx().device];
     require(base != nullptr, "goal_asm_jacobian_noalloc: scratch address not set up for this device");
     a.save_state = save_states ? 1 : 0;

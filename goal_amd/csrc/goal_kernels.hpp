@@ -61,14 +61,25 @@ struct AsmArgs {
   int64_t group_lo, group_hi;
   // J2 (all kinds but ASM_LOCALIZE): the point states, written by a
   // thread-per-point pass before the element pass reads them.  Per point
-  // (e * nq + q): the generalized stress [P, r1, s3] (13) and, for the Jacobian
-  // kinds, the point tangent D (13 x 13, row-major), all times w dv.
+  // p = e * nq + q, PS values, all times w dv: the generalized stress
+  // [P, r1, s3] (13) and, for the Jacobian kinds, the nonzero entries of the
+  // point tangent D (13 x 13; rows P, r1, s3; columns F, p, grad p):
+  //   13 + 13 c + r   D[r][c], c < 9 (the F directions, every row)
+  //   130 + r         D[r][9], r < 10 (the p column; its s3 rows are zero)
+  //   140 + 3 I + j   D[10 + j][10 + I] (grad p acts on s3 only)
+  // Blocked by 64 points so that a wave's store of one value is one
+  // contiguous 512-byte run: value k of point p sits at point_state_index.
   double* pt;
 };
 
 // Doubles per point in AsmArgs::pt: residual kinds, Jacobian kinds.
 constexpr int kPointStress = 13;
-constexpr int kPointTangent = 13 + 169;
+constexpr int kPointTangent = 13 + 117 + 10 + 9;
+__host__ __device__ inline int64_t point_state_index(int64_t p, int k, int ps) {
+  return ((p >> 6) * ps + k) * 64 + (p & 63);
+}
+// Length of AsmArgs::pt for npts points: whole blocks of 64.
+inline int64_t point_state_doubles(int64_t npts, int ps) { return (npts + 63) / 64 * 64 * ps; }
 int point_stride(int kind);
 bool uses_point_pass(int model, int kind);
 

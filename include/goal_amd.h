@@ -283,7 +283,7 @@ int goal_asm_jacobian(goal_mech m, goal_solinfo s, int adjoint, int save_states)
  * residuals and matrices between the two passes live in a 64 MiB array that the
  * library holds on each GPU until the process ends, so a mesh may have at most
  * 5,115 ten-node or 30,840 four-node tets (J2, whose point states take room
- * too: 3,542 ten-node four-point or 18,477 four-node one-point tets; larger
+ * too: 3,748 ten-node four-point or 19,904 four-node one-point tets; larger
  * meshes: GOAL_ERR_INVALID, and nothing is launched).  Results are bitwise those of goal_asm_jacobian.  All
  * calls on one GPU share the array: calls that overlap on one GPU (other
  * streams or threads) race.  Under capture, set the mechanics' "check" option
