@@ -112,7 +112,7 @@ template <int NN, int NQ, int G>
 struct SBase {
   double x[G][4 * NN];
   double X[G][4][3];
-  double sh[G][NQ][NN][4];  // N_a, dN_a/dX
+  double sh[G][NQ][NN][kShapeRow];  // N_a, dN_a/dX, pad
   double wdv[G][NQ];
   double sig[G][NQ][13];
   int32_t conn[G][NN];
@@ -226,7 +226,7 @@ __device__ inline void store_column(const T (*acc)[4], int col, const int32_t* r
 // T: the element-matrix type (double; float for the FP32 tangent, which forms
 // and accumulates K in FP32 from the FP64 point tangent).
 template <int NN, int NQ, bool TR, typename T>
-__device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[13][13], int col,
+__device__ inline void emit_column(const double (*sh)[NN][kShapeRow], const double (*D)[13][13], int col,
                                    const int32_t* rec, T* em) {
   const int b = col >> 2, kc = col & 3;
   T acc[NN][4];
@@ -259,7 +259,7 @@ __device__ inline void emit_column(const double (*sh)[NN][4], const double (*D)[
 // neo-Hookean column (b, kc): C_q from the closed-form tangent contracted
 // with node b's shape function (dev::nh_direction), then the rows as above.
 template <int NN, int NQ, bool TR, typename T>
-__device__ inline void emit_column_nh(const double (*sh)[NN][4], const double* wdv,
+__device__ inline void emit_column_nh(const double (*sh)[NN][kShapeRow], const double* wdv,
                                       const QPStore<MODEL_NEOHOOKEAN, T>* qs, int col, const int32_t* rec,
                                       T* em) {
   const int b = col >> 2, kc = col & 3;
@@ -339,7 +339,7 @@ struct MfTiles {
 };
 
 template <int NN, bool TR, typename T>
-__device__ inline void emit_tile_mfma(const double (*sh)[NN][4], const double* wdv,
+__device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const double* wdv,
                                       const QPStore<MODEL_NEOHOOKEAN, T>* qs, const int32_t* rec, T* em, int lane,
                                       int t) {
   constexpr int ND = 4 * NN;

@@ -523,8 +523,9 @@ void run_chunked(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool wa
     while (r1 < nr && d->range_last_elem[r1] < elem_end) ++r1;
     if (c == K - 1) r1 = nr;
     if (r1 > r0) {
-      HIPCHK(hipEventRecord(ctx().chunk_ev[c], stream()));
-      HIPCHK(hipStreamWaitEvent(ctx().side, ctx().chunk_ev[c], 0));
+      // the wait takes the event's latest record, so the events can cycle
+      HIPCHK(hipEventRecord(ctx().chunk_ev[c % 64], stream()));
+      HIPCHK(hipStreamWaitEvent(ctx().side, ctx().chunk_ev[c % 64], 0));
       g.num_ranges = r1 - r0;
       g.ranges = d->ranges.p + r0;
       HIPCHK(goal::launch_gather(d->mesh.order, g, goal::gather_grid(d->mesh.order, g.em32 != nullptr, d->gather_cap,
@@ -1831,7 +1832,7 @@ int goal_mech_set_option(goal_mech m, const char* key, int64_t value) {
       require(value == 32 || value == 64, "tangent must be 32 or 64");
       m->tangent = (int)value;
     } else if (k == "overlap") {
-      require(value >= -1 && value <= 64, "overlap must be -1 (automatic) or 0..64 chunks");
+      require(value >= -1 && value <= 1024, "overlap must be -1 (automatic) or 0..1024 chunks");
       m->overlap = (int)value;
     } else if (k == "overlap_blocks") {
       require(value >= 0 && value < (1 << 30), "overlap_blocks out of range");

@@ -72,6 +72,11 @@ struct AsmArgs {
   double* pt;
 };
 
+// Doubles per row of the element kernel's LDS shape table (N, dN/dX and one
+// pad): with a 40-byte row stride the rows that one wave reads at once fall on
+// different LDS banks (a 32-byte stride puts every eighth row on the same one).
+constexpr int kShapeRow = 5;
+
 // Doubles per point in AsmArgs::pt: residual kinds, Jacobian kinds.
 constexpr int kPointStress = 13;
 constexpr int kPointTangent = 13 + 117 + 10 + 9;

@@ -253,10 +253,10 @@ int goal_mech_destroy(goal_mech m);
  * at worst),
  * "timing" (1: record device events around the element and gather kernels of
  * each assembly, read back with goal_mech_timing),
- * "overlap" (neo-Hookean only: > 1 runs the element pass in this many chunks
- * and gathers each finished chunk's node ranges on a second stream, bitwise
- * equal to one pass; 0 off; -1, the default, is one pass; not used under
- * stream capture),
+ * "overlap" (neo-Hookean only: 2..1024 runs the element pass in this many
+ * chunks and gathers each finished chunk's node ranges on a second stream,
+ * bitwise equal to one pass; 0 off; -1, the default, is one pass; not used
+ * under stream capture),
  * "overlap_blocks" (element workgroups per chunk, 0 = 5/6 of the resident
  * grid). */
 int goal_mech_set_option(goal_mech m, const char* key, int64_t value);

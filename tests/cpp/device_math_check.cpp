@@ -237,7 +237,7 @@ void check_indices(int NN, int NQ, int G, int T, bool mf) {
       for (int t = tid; t < nv * ND; t += T) in((t / ND) * ND + t % ND, G * ND, "A dofs");
       for (int t = tid; t < nv * NQ * NN; t += T) {
         const int el = t / (NQ * NN), r = t % (NQ * NN);
-        in(((el * NQ + r / NN) * NN + r % NN) * 4 + 3, G * NQ * NN * 4, "B shape table");
+        in(((el * NQ + r / NN) * NN + r % NN) * kShapeRow + 3, G * NQ * NN * kShapeRow, "B shape table");
       }
       for (int t = tid; t < nv * NQ; t += T) in((t / NQ) * NQ + t % NQ, G * NQ, "C1 point");
       for (int t = tid; t < nv * NQ * 9; t += T) {  // C2: D[el][q][rr][c], rr < 13, c < 9
@@ -247,7 +247,7 @@ void check_indices(int NN, int NQ, int G, int T, bool mf) {
       for (int t = tid; t < nv * ND; t += T) {  // D: sig[el][q][..], sh[el][q][n][..]
         const int el = t / ND, k = t % ND, n = k >> 2;
         in((el * NQ + NQ - 1) * 13 + 12, G * NQ * 13, "D stress");
-        in(((el * NQ + NQ - 1) * NN + n) * 4 + 3, G * NQ * NN * 4, "D shape table");
+        in(((el * NQ + NQ - 1) * NN + n) * kShapeRow + 3, G * NQ * NN * kShapeRow, "D shape table");
       }
       if (!mf) {
         for (int t = tid; t < nv * ND; t += T) {  // E: one column per lane
@@ -274,9 +274,42 @@ void check_indices(int NN, int NQ, int G, int T, bool mf) {
     }
 }
 
+// The J2 point states: the element kernel's read of value r (r >= 13) and its
+// fill of the tangent's structural zeros together cover every entry of the
+// 13 x 13 table exactly once, and the blocked index of every value of every
+// point stays inside the buffer point_state_doubles sizes.
+void check_point_states() {
+  int hits[13][13] = {};
+  for (int r = 13; r < kPointTangent; ++r) {
+    int row, col;
+    if (r < 130) row = (r - 13) % 13, col = (r - 13) / 13;
+    else if (r < 140) row = r - 130, col = 9;
+    else row = 10 + (r - 140) % 3, col = 10 + (r - 140) / 3;
+    check(row >= 0 && row < 13 && col >= 0 && col < 13, "point tangent position", row, col);
+    if (row >= 0 && row < 13 && col >= 0 && col < 13) ++hits[row][col];
+  }
+  for (int z = 0; z < 33; ++z) {
+    const int row = z < 3 ? 10 + z : (z - 3) % 10, col = z < 3 ? 9 : 10 + (z - 3) / 10;
+    ++hits[row][col];
+  }
+  for (int i = 0; i < 13; ++i)
+    for (int j = 0; j < 13; ++j) check(hits[i][j] == 1, "point tangent entry covered once", i * 13 + j, hits[i][j]);
+  for (long npts : {1L, 63L, 64L, 65L, 1000L}) {
+    for (int ps : {kPointStress, kPointTangent}) {
+      const long n = (long)point_state_doubles(npts, ps);
+      for (long p = 0; p < npts; ++p)
+        for (int k = 0; k < ps; ++k) {
+          const long idx = (long)point_state_index(p, k, ps);
+          if (idx < 0 || idx >= n) check(false, "point state index", idx, n);
+        }
+    }
+  }
+}
+
 }  // namespace
 
 int main() {
+  check_point_states();
   check_j2(200, true);
   check_j2(50, false);
   check_nh(100);
