@@ -115,4 +115,96 @@ bool dense_inverse(int n, std::vector<double>& a) {
   return true;
 }
 
+std::vector<std::complex<double>> hessenberg_eigenvalues(int n, const std::vector<double>& h) {
+  using C = std::complex<double>;
+  std::vector<C> a((size_t)n * n);
+  for (size_t k = 0; k < a.size(); ++k) a[k] = h[k];
+  auto A = [&](int i, int j) -> C& { return a[(size_t)i * n + j]; };
+  std::vector<C> ev(n);
+  std::vector<double> gc(n);
+  std::vector<C> gs(n);
+  int hi = n - 1, its = 0;
+  while (hi >= 0) {
+    if (hi == 0) {
+      ev[0] = A(0, 0);
+      break;
+    }
+    // the active block [lo, hi]: every subdiagonal inside it is non-negligible
+    int lo = hi;
+    while (lo > 0) {
+      const double scale = std::abs(A(lo - 1, lo - 1)) + std::abs(A(lo, lo));
+      if (std::abs(A(lo, lo - 1)) <= 1e-15 * (scale > 0.0 ? scale : 1.0)) break;
+      --lo;
+    }
+    if (lo == hi || its >= 200) {  // deflate (or give up on this one after 200 steps)
+      ev[hi] = A(hi, hi);
+      --hi;
+      its = 0;
+      continue;
+    }
+    // Wilkinson shift: the eigenvalue of the trailing 2 x 2 nearer A(hi, hi)
+    const C p = A(hi - 1, hi - 1), q = A(hi - 1, hi), r = A(hi, hi - 1), t = A(hi, hi);
+    const C half = 0.5 * (p - t), disc = std::sqrt(half * half + q * r);
+    C mu = t - q * r / (half + (std::abs(half + disc) >= std::abs(half - disc) ? disc : -disc));
+    if (!std::isfinite(mu.real()) || !std::isfinite(mu.imag())) mu = t;
+    if (its % 11 == 10) mu += std::abs(A(hi, hi - 1)) * 0.75;  // exceptional shift
+    ++its;
+    for (int k = lo; k <= hi; ++k) A(k, k) -= mu;
+    // A - mu = QR by Givens rotations on the subdiagonal, then A = RQ + mu
+    for (int k = lo; k < hi; ++k) {
+      const C x = A(k, k), y = A(k + 1, k);
+      const double ax = std::abs(x), nr = std::hypot(ax, std::abs(y));
+      double c;
+      C sg;
+      if (nr == 0.0) {
+        c = 1.0;
+        sg = 0.0;
+      } else if (ax == 0.0) {
+        c = 0.0;
+        sg = 1.0;
+      } else {
+        c = ax / nr;
+        sg = (x / ax) * std::conj(y) / nr;
+      }
+      gc[k] = c;
+      gs[k] = sg;
+      for (int j = k; j <= hi; ++j) {
+        const C u = A(k, j), v = A(k + 1, j);
+        A(k, j) = c * u + sg * v;
+        A(k + 1, j) = -std::conj(sg) * u + c * v;
+      }
+    }
+    for (int k = lo; k < hi; ++k) {
+      const double c = gc[k];
+      const C sg = gs[k];
+      for (int i = lo; i <= std::min(k + 2, hi); ++i) {
+        const C u = A(i, k), v = A(i, k + 1);
+        A(i, k) = c * u + std::conj(sg) * v;
+        A(i, k + 1) = -sg * u + c * v;
+      }
+    }
+    for (int k = lo; k <= hi; ++k) A(k, k) += mu;
+  }
+  return ev;
+}
+
+void chebyshev_ellipse(const std::vector<std::complex<double>>& ritz, double* theta, double* c2) {
+  double top = 0.0;
+  for (const auto& z : ritz) top = std::max(top, std::max(z.real(), z.real() > 0.0 ? std::abs(z) : 0.0));
+  if (!(top > 0.0) || !std::isfinite(top)) {
+    *theta = 0.0;
+    *c2 = 0.0;
+    return;
+  }
+  const double hi = 1.1 * top, lo = hi / 10.0, d = 0.5 * (hi + lo), a = 0.5 * (hi - lo);
+  double b = 0.0;
+  for (const auto& z : ritz) {
+    const double u = (z.real() - d) / a;
+    if (std::fabs(u) < 1.0) b = std::max(b, std::fabs(z.imag()) / std::sqrt(1.0 - u * u));
+  }
+  b *= 1.1;
+  *theta = d;
+  *c2 = a * a - b * b;
+}
+
 }  // namespace goal

@@ -8,6 +8,7 @@
 // inputs): aggregates of nodes, each node's 4 equations kept together.
 #pragma once
 
+#include <complex>
 #include <cstdint>
 #include <vector>
 
@@ -37,5 +38,19 @@ AmgCoarsening amg_coarsen(int64_t nf, const int64_t* fptr, const int32_t* fcol);
 // In-place Gauss-Jordan inverse of a dense n x n row-major matrix with
 // partial pivoting.  Returns false (and leaves the identity) if singular.
 bool dense_inverse(int n, std::vector<double>& a);
+
+// Eigenvalues of a real upper Hessenberg n x n matrix (row-major, n small:
+// the Arnoldi matrix of a smoother's spectrum estimate) by the shifted QR
+// algorithm in complex arithmetic: Givens rotations, Wilkinson shifts,
+// deflation at negligible subdiagonal entries.
+std::vector<std::complex<double>> hessenberg_eigenvalues(int n, const std::vector<double>& h);
+
+// The Chebyshev smoother's ellipse from Ritz values of Binv A: real extent
+// [hi / 10, hi] with hi = 1.1 max(Re, |.|) over the Ritz values, imaginary
+// semi-axis just large enough (times 1.1) to enclose the Ritz values whose
+// real part lies inside that extent.  Returns the centre theta and the squared
+// focal half-distance c2 = a^2 - b^2 (negative when the ellipse is taller
+// than wide); theta = 0 when no Ritz value has a positive real part.
+void chebyshev_ellipse(const std::vector<std::complex<double>>& ritz, double* theta, double* c2);
 
 }  // namespace goal

@@ -211,12 +211,13 @@ struct AmgLevel {
   DevBuf<int32_t> agg, amem;
   DevBuf<int64_t> aptr, gptr, gidx;
   DevBuf<double> binv, b, x, r, d;        // smoother inverse blocks, work vectors
-  double lmax = 1.0;                      // spectral radius estimate of Binv A
+  double theta = 1.0, c2 = 0.0;           // smoother ellipse: centre, squared focal half-distance
 };
 struct AmgHierarchy {
   std::vector<std::unique_ptr<AmgLevel>> lv;
   DevBuf<int32_t> brow0;                  // row node of every finest-level block
   DevBuf<double> coarse_inv;              // dense inverse of the coarsest level
+  DevBuf<double> arn;                     // Arnoldi basis of the smoother's spectrum estimate
   int coarse_n = 0;
 };
 
@@ -624,17 +625,23 @@ void run_assembly(goal_mech m, goal_solinfo s, int kind, goal::AsmArgs a, bool w
 //     matrices the Galerkin sums P^T A P, down to at most kAmgCoarseNodes nodes,
 //     whose matrix is inverted densely.
 //   - Smoother: Chebyshev of degree kAmgDegree on Binv A (Binv: inverse 4x4
-//     diagonal blocks), on [lmax / 10, 1.1 lmax] with lmax from a power
-//     iteration, before and after the coarse correction.  (A numpy model of
-//     this hierarchy on the P2 cube's and the plastic J2 bar's Jacobians:
-//     degree 3 on [lmax/30, ..] took 63 and 132-178 GMRES iterations to 1e-8,
-//     degree 4 on [lmax/10, ..] 57 and 69-92; block Jacobi alone 800+ and
-//     did not converge in 1,500-2,000.)
+//     diagonal blocks), before and after the coarse correction, on an ellipse
+//     around the spectrum: real extent [hi / 10, hi], imaginary semi-axis
+//     from the Ritz values of kAmgArnoldi Arnoldi steps on Binv A
+//     (goal::chebyshev_ellipse).  The plastic J2 tangent makes Binv A
+//     strongly non-normal (eigenvalues such as 0.46 +- 0.76i next to a
+//     largest real one of 2.1), which Chebyshev on the real interval alone
+//     amplifies.  A numpy model of this hierarchy: on a 446 K-tet graded
+//     bar at a plastic stretch, GMRES stalled at a relative residual of 0.24
+//     after 1,500 iterations with the interval and reached 1e-8 in 132 with
+//     the ellipse; on the Newton-state 65 K-tet bar 75 and 80 iterations; on
+//     the P2 cube's and plastic bar's Jacobians degree 4 on [lmax/10, ..]
+//     took 57 and 69-92, block Jacobi alone 800+ without converging.
 // Every kernel sums in a fixed order, so a solve is bitwise reproducible.
 constexpr int kAmgCoarseNodes = 128;
 constexpr int kAmgMaxLevels = 12;
 constexpr int kAmgDegree = 4;
-constexpr int kAmgPowerIters = 12;
+constexpr int kAmgArnoldi = 10;
 
 void amg_build_structure(goal_disc d) {
   auto h = std::make_unique<AmgHierarchy>();
@@ -687,6 +694,7 @@ void amg_build_structure(goal_disc d) {
   }
   const int64_t nc = h->lv.back()->nn;
   h->coarse_n = (int)(4 * nc);
+  h->arn.alloc((int64_t)(kAmgArnoldi + 1) * 4 * nn0);
   h->coarse_inv.alloc((int64_t)h->coarse_n * h->coarse_n);
   d->amg = std::move(h);
 }
@@ -720,26 +728,51 @@ void amg_setup(goal_solinfo s, int* status) {
       HIPCHK(goal::launch_bsr_block_inverse(L.nn, L.bptr.p, L.bcol.p, L.bval.p, L.binv.p, status, stream()));
     }
     if (l + 1 == nl) break;
-    // spectral radius of Binv A by power iteration from a fixed start vector
-    std::vector<double> v0(4 * L.nn);
-    for (int64_t i = 0; i < 4 * L.nn; ++i)
+    // Arnoldi on Binv A from a fixed start vector (classical Gram-Schmidt,
+    // two passes); its Ritz values place the smoother's ellipse
+    const int64_t nv = 4 * L.nn;
+    double* Q = h.arn.p;
+    std::vector<double> v0(nv);
+    for (int64_t i = 0; i < nv; ++i)
       v0[i] = (double)(goal::splitmix64(0xA3C5u + (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
-    HIPCHK(hipMemcpyAsync(L.x.p, v0.data(), sizeof(double) * v0.size(), hipMemcpyHostToDevice, stream()));
-    double lam = 1.0;
-    for (int it = 0; it < kAmgPowerIters; ++it) {
-      amg_residual(s, h, l, L.x.p, nullptr, L.r.p);
-      HIPCHK(goal::launch_block_apply(L.nn, L.binv.p, L.r.p, L.d.p, stream()));
-      HIPCHK(goal::launch_reduce(1, L.d.p, nullptr, 4 * L.nn, reduce_scratch(), result_slot(3), stream()));
-      HIPCHK(goal::launch_reduce(1, L.x.p, nullptr, 4 * L.nn, reduce_scratch(), result_slot(4), stream()));
-      double nd = 0.0, nx = 0.0;
-      HIPCHK(hipMemcpyAsync(&nd, result_slot(3), sizeof(double), hipMemcpyDeviceToHost, stream()));
-      HIPCHK(hipMemcpyAsync(&nx, result_slot(4), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    HIPCHK(hipMemcpyAsync(Q, v0.data(), sizeof(double) * nv, hipMemcpyHostToDevice, stream()));
+    auto norm2 = [&](const double* v) {  // waits
+      HIPCHK(goal::launch_reduce(1, v, nullptr, nv, reduce_scratch(), result_slot(3), stream()));
+      double r2 = 0.0;
+      HIPCHK(hipMemcpyAsync(&r2, result_slot(3), sizeof(double), hipMemcpyDeviceToHost, stream()));
       sync();
-      if (!(nd > 0.0) || !(nx > 0.0)) break;
-      lam = std::sqrt(nd / nx);
-      HIPCHK(goal::launch_scale(4 * L.nn, 1.0 / std::sqrt(nd), L.d.p, L.x.p, stream()));
+      return std::sqrt(r2);
+    };
+    const double q0 = norm2(Q);
+    HIPCHK(goal::launch_scale(nv, 1.0 / q0, Q, Q, stream()));
+    constexpr int M = kAmgArnoldi;
+    std::vector<double> H((size_t)(M + 1) * M, 0.0), hc(M + 1);
+    int steps = 0;
+    for (int j = 0; j < M; ++j) {
+      double* w = Q + (int64_t)(j + 1) * nv;
+      amg_residual(s, h, l, Q + (int64_t)j * nv, nullptr, L.r.p);
+      HIPCHK(goal::launch_block_apply(L.nn, L.binv.p, L.r.p, w, stream()));
+      for (int pass = 0; pass < 2; ++pass) {
+        HIPCHK(goal::launch_multi_dot(nv, j + 1, Q, nv, w, s->partial.p, s->h1.p, stream()));
+        HIPCHK(goal::launch_multi_axpy(nv, j + 1, Q, nv, s->h1.p, -1.0, w, stream()));
+        HIPCHK(hipMemcpyAsync(hc.data(), s->h1.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, stream()));
+        sync();
+        for (int i = 0; i <= j; ++i) H[(size_t)i * M + j] += hc[i];
+      }
+      const double hn = norm2(w);
+      H[(size_t)(j + 1) * M + j] = hn;
+      steps = j + 1;
+      if (!(hn > 1e-12 * std::fabs(H[(size_t)j * M + j])) || !std::isfinite(hn)) break;  // invariant subspace
+      HIPCHK(goal::launch_scale(nv, 1.0 / hn, w, w, stream()));
     }
-    L.lmax = lam;
+    std::vector<double> Hs((size_t)steps * steps);
+    for (int i = 0; i < steps; ++i)
+      for (int j = 0; j < steps; ++j) Hs[(size_t)i * steps + j] = H[(size_t)i * M + j];
+    goal::chebyshev_ellipse(goal::hessenberg_eigenvalues(steps, Hs), &L.theta, &L.c2);
+    if (!(L.theta > 0.0)) {  // no Ritz value with a positive real part: plain block Jacobi steps
+      L.theta = 1.0;
+      L.c2 = 0.0;
+    }
   }
   // the coarsest level's matrix, densely inverted on the host
   AmgLevel& C = *h.lv[nl - 1];
@@ -774,9 +807,13 @@ void amg_setup(goal_solinfo s, int* status) {
 // x = Chebyshev smoothing of A_l x = b (from x = 0 when zero).
 void amg_smooth(goal_solinfo s, AmgHierarchy& h, int l, const double* b, double* x, bool zero) {
   AmgLevel& L = *h.lv[l];
-  const double hi = 1.1 * L.lmax, lo = hi / 10.0;
-  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-  double rho = 1.0 / sigma;
+  // Chebyshev iteration for the ellipse with centre theta and foci
+  // theta +- sqrt(c2): with kappa_0 = 1 / theta and kappa_{k+1} =
+  // 1 / (2 theta - c2 kappa_k), d_0 = kappa_0 Binv r_0 and d_{k+1} =
+  // c2 kappa_{k+1} kappa_k d_k + 2 kappa_{k+1} Binv r_{k+1}, all real also
+  // when the foci are imaginary (c2 < 0)
+  const double th = L.theta, c2 = L.c2;
+  double k0 = 1.0 / th;
   const double* r = b;
   if (zero) {
     HIPCHK(goal::launch_fill(x, 4 * L.nn, 0.0, stream()));
@@ -784,12 +821,12 @@ void amg_smooth(goal_solinfo s, AmgHierarchy& h, int l, const double* b, double*
     amg_residual(s, h, l, x, b, L.r.p);
     r = L.r.p;
   }
-  HIPCHK(goal::launch_cheb(L.nn, L.binv.p, r, 0.0, 1.0 / theta, L.d.p, x, stream()));
+  HIPCHK(goal::launch_cheb(L.nn, L.binv.p, r, 0.0, k0, L.d.p, x, stream()));
   for (int k = 1; k < kAmgDegree; ++k) {
-    const double rn = 1.0 / (2.0 * sigma - rho);
+    const double k1 = 1.0 / (2.0 * th - c2 * k0);
     amg_residual(s, h, l, x, b, L.r.p);
-    HIPCHK(goal::launch_cheb(L.nn, L.binv.p, L.r.p, rn * rho, 2.0 * rn / delta, L.d.p, x, stream()));
-    rho = rn;
+    HIPCHK(goal::launch_cheb(L.nn, L.binv.p, L.r.p, c2 * k1 * k0, 2.0 * k1, L.d.p, x, stream()));
+    k0 = k1;
   }
 }
 
@@ -1632,8 +1669,9 @@ int goal_linear_solve_pc(goal_solinfo s, const double* b, double* x, double rtol
     if (s->z.n < n) s->z.alloc(n);
     if (s->binv.n < 16 * nn) s->binv.alloc(16 * nn);
     const int nb = goal::multi_dot_blocks(n);
-    if (s->partial.n < (int64_t)(m + 2) * nb) s->partial.alloc((int64_t)(m + 2) * nb);
-    if (s->h1.n < m + 2) s->h1.alloc(m + 2);
+    const int mk = std::max(m, kAmgArnoldi);  // the multigrid set-up's Arnoldi shares these
+    if (s->partial.n < (int64_t)(mk + 2) * nb) s->partial.alloc((int64_t)(mk + 2) * nb);
+    if (s->h1.n < mk + 2) s->h1.alloc(mk + 2);
     if (s->h2.n < m + 1) s->h2.alloc(m + 1);
     if (s->y.n < m + 1) s->y.alloc(m + 1);
     if (s->solve_status.n < 1) s->solve_status.alloc(1);
