@@ -927,7 +927,7 @@ __global__ __launch_bounds__(kGatherThreads) void gather_kernel(GatherArgs g) {
 // budget of the return map; here every lane takes a point and no barrier
 // holds it.  The arithmetic is that of phases B, C1 and C2, in the same order.
 template <int NN, int NQ>
-__global__ __launch_bounds__(256) void j2_point_kernel(AsmArgs a, int jac) {
+__global__ __launch_bounds__(256, 2) void j2_point_kernel(AsmArgs a, int jac) {
   const int64_t npts = a.num_elems * NQ, nqtot = npts;
   const int PS = jac ? kPointTangent : kPointStress;
   for (int64_t sidx = (int64_t)blockIdx.x * 256 + threadIdx.x; sidx < npts; sidx += (int64_t)gridDim.x * 256) {
@@ -997,9 +997,11 @@ __global__ __launch_bounds__(256) void j2_point_kernel(AsmArgs a, int jac) {
       st.dir_g(qp, I, ds3);
       for (int j = 0; j < 3; ++j) o[64 * (140 + 3 * I + j)] = ds3[j] * wdv;
     }
-    // F columns: directional derivatives through the return map (unrolled:
-    // the direction indexes the point's 3x3 arrays, which registers cannot)
-#pragma unroll
+    // F columns: directional derivatives through the return map, one
+    // direction at a time (the derivatives pick the direction's rows and
+    // columns by selects, so the loop stays rolled and the registers hold
+    // one direction's temporaries)
+#pragma unroll 1
     for (int c = 0; c < 9; ++c) {
       dev::Dir d;
       m.dir(qp, c / 3, c % 3, d);

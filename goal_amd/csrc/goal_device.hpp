@@ -173,6 +173,13 @@ struct QP {
 };
 
 // Output of one directional derivative.
+// v0, v1 or v2 by a runtime index without indexing a register array (the
+// directional derivatives below pick rows and columns of the point's 3 x 3
+// arrays this way, so a loop over directions need not be unrolled)
+__host__ __device__ inline double sel3(int i, double v0, double v1, double v2) {
+  return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+
 struct Dir {
   double dsig[9];  // Cauchy stress after Mixed (QoI derivatives)
   double dP[9];
@@ -198,11 +205,17 @@ struct Stab {
   // direction dF = e_k e_L^T:
   //   dJ = J Finv_Lk,  dCinv_IJ = -Finv_Ik Cinv_LJ - Finv_Jk Cinv_IL
   __host__ __device__ inline void dir(const QP& q, int k, int L, double* ds3) const {
-    double dJJ = q.Fi[3 * L + k];  // dJ / J
+    double FiK[3], CL[3], CcL[3];  // column k of Finv, row L and column L of Cinv
+    for (int m = 0; m < 3; ++m) {
+      FiK[m] = sel3(k, q.Fi[3 * m], q.Fi[3 * m + 1], q.Fi[3 * m + 2]);
+      CL[m] = sel3(L, Cinv[m], Cinv[3 + m], Cinv[6 + m]);
+      CcL[m] = sel3(L, Cinv[3 * m], Cinv[3 * m + 1], Cinv[3 * m + 2]);
+    }
+    double dJJ = sel3(L, FiK[0], FiK[1], FiK[2]);  // Finv_Lk = dJ / J
     for (int j = 0; j < 3; ++j) {
       double acc = 0.0;
       for (int I = 0; I < 3; ++I) {
-        double dC = -q.Fi[3 * I + k] * Cinv[3 * L + j] - q.Fi[3 * j + k] * Cinv[3 * I + L];
+        double dC = -FiK[I] * CL[j] - FiK[j] * CcL[I];
         acc += dC * q.g[I];
       }
       ds3[j] = tau * (dJJ * JCg[j] + q.J * acc);
@@ -217,11 +230,16 @@ struct Stab {
 // dP for a direction of F given sigma, dsigma:
 //   dP_iJ = (dJ/J) P_iJ + J (dsigma F^{-T})_iJ - Finv_Jk P_iL
 __host__ __device__ inline void dpiola(const QP& q, const double* P, const double* dsig, int k, int L, double* dP) {
-  double dJJ = q.Fi[3 * L + k];
+  double FiK[3], PL[3];  // column k of Finv, column L of P
+  for (int m = 0; m < 3; ++m) {
+    FiK[m] = sel3(k, q.Fi[3 * m], q.Fi[3 * m + 1], q.Fi[3 * m + 2]);
+    PL[m] = sel3(L, P[3 * m], P[3 * m + 1], P[3 * m + 2]);
+  }
+  double dJJ = sel3(L, FiK[0], FiK[1], FiK[2]);
   for (int i = 0; i < 3; ++i)
     for (int c = 0; c < 3; ++c) {
       double t = dsig[3 * i] * q.Fi[3 * c] + dsig[3 * i + 1] * q.Fi[3 * c + 1] + dsig[3 * i + 2] * q.Fi[3 * c + 2];
-      dP[3 * i + c] = dJJ * P[3 * i + c] + q.J * t - q.Fi[3 * c + k] * P[3 * i + L];
+      dP[3 * i + c] = dJJ * P[3 * i + c] + q.J * t - FiK[c] * PL[i];
     }
 }
 
@@ -375,13 +393,15 @@ struct J2Model {
   // dF = e_k e_L^T with bebar = F Cp^{-1} F^T and G = F Cp^{-1}:
   //   d bebar_ij = d_ik G_jL + G_iL d_jk, with G_jL = (bebar F^{-T})_jL
   __host__ __device__ inline void dir(const QP& q, int k, int L, Dir& d) const {
-    double dJJ = q.Fi[3 * L + k];
+    double FiL[3];  // row L of Finv
+    for (int m = 0; m < 3; ++m) FiL[m] = sel3(L, q.Fi[m], q.Fi[3 + m], q.Fi[6 + m]);
+    double dJJ = sel3(k, FiL[0], FiL[1], FiL[2]);
     double dJ = q.J * dJJ;
     double dJm23 = -(2.0 / 3.0) * Jm23 * dJJ;
     // G = F Cp^{-1} recovered from bebar: G = bebar F^{-T}
     double GL[3];
     for (int j = 0; j < 3; ++j)
-      GL[j] = bebar[3 * j] * q.Fi[3 * L] + bebar[3 * j + 1] * q.Fi[3 * L + 1] + bebar[3 * j + 2] * q.Fi[3 * L + 2];
+      GL[j] = bebar[3 * j] * FiL[0] + bebar[3 * j + 1] * FiL[1] + bebar[3 * j + 2] * FiL[2];
     double dbe[9];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {

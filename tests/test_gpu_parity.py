@@ -420,6 +420,7 @@ def test_multigrid_smoother_handles_plastic_bar():
     A = sp.csr_matrix((m.si.vals(), cols, rp), shape=(n, n))
     rhs = -m.si.R()
     b, y = g.DeviceArray.from_host(rhs), g.DeviceArray(n)
+    y.fill(0.0)  # the initial guess (the relative residual is against b - A x0)
     it, rr = m.si.solve(b, y, rtol=1e-8, max_iters=1500, restart=100, precond=g.PRECOND_AMG)
     print("graded plastic bar: %d tets, %d GMRES iterations, relres %.2e" % (m.ne, it, rr))
     assert rr <= 1e-8 and it <= 600
