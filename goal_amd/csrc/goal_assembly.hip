@@ -423,6 +423,26 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
   }
   const int64_t nqtot = ne * NQ;
   bool fetched = false;  // this group's phase-A data is in the prefetch buffers
+  // J2 point-pass kinds: four-node tets load a group's point states into
+  // registers one group ahead, so the loads are in flight during the previous
+  // group's phases (ten-node tets have no registers to spare for it)
+  constexpr bool PP = kPointPass<MODEL, KIND>;
+  constexpr int PPS = PP ? (JAC ? kPointTangent : kPointStress) : 1;
+  constexpr int NPF = PP ? (G * NQ * PPS + T - 1) / T : 1;
+  constexpr bool PPF = PP && NN == 4 && NQ == 1;
+  double pf[PPF ? NPF : 1];
+  auto fetch_points = [&](int64_t it_) {
+    const int64_t gq = group_of(it_, ngroups, a.xcd_remap);
+    if (gq < 0) return;
+    const int64_t e0q = (g_lo + gq) * G;
+    const int npq = (int)min((int64_t)G, ne - e0q) * NQ;
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+      const int t = tid + i * T;
+      if (t < npq * PPS) pf[i] = a.pt[point_state_index(e0q * NQ + t % npq, t / npq, PPS)];
+    }
+  };
+  if constexpr (PPF) fetch_points(0);
   if constexpr (kPointPass<MODEL, KIND> && JAC) {
     // the point tangent's structural zeros (s3 rows of the p column, P and r1
     // rows of the grad-p columns): never stored, never overwritten
@@ -569,13 +589,13 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
         }
       }
     }
-    if constexpr (kPointPass<MODEL, KIND>) {
-      // the point pass left [P, r1, s3] and the tangent, times w dv, per point
-      constexpr int PS = JAC ? kPointTangent : kPointStress;
+    if constexpr (PP) {
+      // the point pass left [P, r1, s3] and the tangent, times w dv, per
+      // point (with PPF they arrived in pf during the previous group)
+      constexpr int PS = PPS;
       const int npq = nv * NQ;
-      for (int t = tid; t < npq * PS; t += T) {
+      auto put = [&](int t, double v) {
         const int r = t / npq, pq = t % npq;  // consecutive lanes: consecutive points
-        const double v = a.pt[point_state_index(e0 * NQ + pq, r, PS)];
         if (r < 13) {
           s.sig[pq / NQ][pq % NQ][r] = v;
         } else if constexpr (JAC) {
@@ -584,7 +604,15 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
           else if (r < 140) D[r - 130][9] = v;
           else D[10 + (r - 140) % 3][10 + (r - 140) / 3] = v;
         }
+      };
+      if constexpr (PPF) {
+#pragma unroll
+        for (int i = 0; i < NPF; ++i)
+          if (tid + i * T < npq * PS) put(tid + i * T, pf[i]);
+      } else {
+        for (int t = tid; t < npq * PS; t += T) put(t, a.pt[point_state_index(e0 * NQ + t % npq, t / npq, PS)]);
       }
+      if constexpr (PPF) fetch_points(it + 1);
     }
     if constexpr (!kPointPass<MODEL, KIND>) {
       for (int t = tid; t < nv * NQ; t += T) {
