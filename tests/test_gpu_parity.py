@@ -560,3 +560,72 @@ def test_chunked_assembly_is_bitwise_equal(order, quad, n, tangent):
         mech.residual(si, save_states=True)
         assert np.array_equal(si.R(), ref_res), chunks
     mech.set_option("overlap", 0)
+
+
+# Parity at scale: meshes with thousands of element groups, so that every
+# workgroup of the resident grid walks several groups, the XCD slab schedule
+# splits the groups over all eight XCDs, the phase-A prefetch and the J2
+# point-state prefetch carry data across group iterations, and the gather
+# covers thousands of node runs.
+SCALE = [
+    # (order, quad, model, dims, lengths, tangent)
+    (2, 2, 1, (16, 16, 16), (1.0, 1.0, 1.0), 64),   # 24,576 tets, 4,096 groups of 6
+    (2, 2, 1, (16, 16, 16), (1.0, 1.0, 1.0), 32),
+    (1, 1, 2, (48, 8, 8), (8.0, 1.0, 1.0), 64),     # 18,432 tets, 1,152 groups of 16
+    (2, 2, 2, (8, 8, 6), (1.0, 1.0, 0.75), 64),     # 2,304 tets, 384 groups of 6, 4 points
+]
+
+
+@pytest.mark.parametrize("order,quad,model,dims,lengths,tangent", SCALE)
+def test_jacobian_at_scale(order, quad, model, dims, lengths, tangent):
+    """R, the CSR values and the stresses of a smooth iterate with small
+    random noise (neo-Hookean) or a 1% plastic stretch over random plastic
+    history (J2) against the oracle: RTOL for FP64, F32_TOL for FP32 element
+    matrices."""
+    disc = g.Disc.box(*dims, lx=lengths[0], ly=lengths[1], lz=lengths[2], order=order, jitter=0.1, seed=77)
+    m = MATS[model]
+    st = g.States(disc, model, quad)
+    mech = g.Mechanics(disc, st, [g.MaterialSpec(**m)], stabilization=True, quad_order=quad)
+    mech.set_option("tangent", tangent)
+    coords, _, _ = disc.mesh()
+    X, Y, Z = coords[:, 0], coords[:, 1], coords[:, 2]
+    rng = np.random.default_rng(5)
+    h = min(lengths[i] / dims[i] for i in range(3))
+    x = np.zeros(disc.num_dofs)
+    if model == 2:
+        x[0::4] = 0.01 * X
+        x[1::4] = -0.003 * Y + 1e-3 * np.sin(np.pi * X / lengths[0]) * Z
+        x[2::4] = -0.003 * Z + 1e-3 * np.cos(np.pi * X / lengths[0]) * Y
+    else:
+        x[0::4] = 0.02 * np.sin(np.pi * X) * np.sin(np.pi * Y) * Z
+        x[1::4] = 0.01 * X * Y * Z
+        x[2::4] = 0.03 * Z * (1.0 + 0.5 * np.sin(np.pi * Y))
+    for c in range(3):
+        x[c::4] += 0.01 * h * rng.standard_normal(disc.num_nodes)
+    x[3::4] = rng.standard_normal(disc.num_nodes)
+    disc.set_soln(x)
+    om = oracle_mesh_from(disc)
+    ost = O.States(om, model, quad)
+    if model == 2:
+        Fp, eq = random_history(disc.num_elems * st.nq, 6)
+        for k, v in (("Fp_old", Fp), ("eqps_old", eq), ("Fp", Fp), ("eqps", eq)):
+            st.set(k, v)
+        ost.Fp_old[:] = Fp
+        ost.eqps_old[:] = eq
+        ost.Fp[:] = Fp
+        ost.eqps[:] = eq
+    si = g.SolInfo(disc)
+    si.zero_all()
+    mech.jacobian(si, adjoint=False, save_states=True)
+    assert mech.status() == 0
+    Ro, Vo = O.assemble(O.K_JACOBIAN, om, model, [mat_row(m)], x, ost, quad_order=quad)
+    assert np.isfinite(Ro).all() and np.isfinite(Vo).all()
+    assert rel(si.R(), Ro) < RTOL
+    assert rel(st.get("sigma"), ost.sigma) < RTOL
+    if model == 2:
+        assert (ost.eqps > eq).any()  # the stretch is past yield somewhere
+        assert rel(st.get("Fp"), ost.Fp) < RTOL
+    e = rel(si.vals(), Vo)
+    assert e < (RTOL if tangent == 64 else F32_TOL)
+    if tangent == 32:
+        assert e > 0.0
