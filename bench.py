@@ -819,7 +819,7 @@ def main() -> int:
     ap.add_argument("--dataset-out", default="", help="learned: save the generated dataset here (every 25 runs)")
     ap.add_argument("--cycles", type=int, default=5)
     ap.add_argument("--final-elems", type=int, default=2_000_000, help="learned: final size of the cube runs")
-    ap.add_argument("--final-elems-bar", type=int, default=200_000, help="learned: final size of the bar runs")
+    ap.add_argument("--final-elems-bar", type=int, default=2_000_000, help="learned: final size of the bar runs")
     ap.add_argument("--eval-seed", type=int, default=2024)
     ap.add_argument("--problems", default="cube,bar", help="learned: which evaluation problems to run")
     ap.add_argument("--blocks", type=int, default=0)

@@ -118,13 +118,29 @@ class Model:
         self.ne = self.conn.shape[0]
         self.newton_info = {}
 
-    def newton(self, t=1.0, tol=1e-10, max_iters=12, lin_rtol=1e-8, lin_max=20000, restart=100, log=None) -> dict:
-        """Primal::solve (src/goal_primal.cpp:111-137) with the device GMRES;
-        log, if given, is called after every iteration."""
+    def newton(self, t=1.0, tol=1e-10, max_iters=30, lin_rtol=1e-8, lin_max=20000, restart=100, log=None,
+               max_halvings=6) -> dict:
+        """Primal::solve (src/goal_primal.cpp:111-137) with the device GMRES and
+        one addition: a backtracking line search on ||R||.  goal takes every
+        full Newton step; here a step that does not lower ||R|| by 1e-4 of its
+        length is halved, up to max_halvings times (the last trial is kept).
+        Full steps that lower ||R|| are taken unchanged, so converging solves
+        are goal's iteration for iteration; on J2 bars refined hard at the
+        clamp, full steps stall.  log, if given, is called after every
+        iteration."""
         n = self.disc.num_dofs
         b, du = g.DeviceArray(n), g.DeviceArray(n)
-        lin, it, norm = [], 0, np.inf
+        lin, it, alphas = [], 0, []
         t0 = time.perf_counter()
+
+        def residual_norm():
+            self.si.zero_R()
+            self.mech.residual(self.si, save_states=True)
+            for comp, ns, expr in self.bcs:
+                g.dbc_apply(self.disc, self.si, comp, ns, expr, t, False)
+            return self.si.norm_R()
+
+        norm = residual_norm()  # the line search's reference; goal always takes a first step
         while it < max_iters:
             self.si.zero_all()
             self.mech.jacobian(self.si, adjoint=False, save_states=True)
@@ -135,20 +151,27 @@ class Model:
             du.fill(0.0)
             k, _ = self.si.solve(b, du, rtol=lin_rtol, max_iters=lin_max, restart=restart)
             lin.append(k)
-            self.disc.add_soln_dev(du)
-            self.si.zero_R()
-            self.mech.residual(self.si, save_states=True)
-            for comp, ns, expr in self.bcs:
-                g.dbc_apply(self.disc, self.si, comp, ns, expr, t, False)
-            norm = self.si.norm_R()
+            self.disc.add_soln_dev(du)            # x0 + du
+            alpha, new = 1.0, residual_norm()
+            for h in range(max_halvings):
+                if new < (1.0 - 1e-4 * alpha) * norm:
+                    break
+                # x0 + alpha du -> x0 + alpha/2 du: add -alpha/2 du
+                g.check(g.lib.goal_device_scale(du.ptr, du.ptr, n, -0.5 if h == 0 else 0.5))
+                self.disc.add_soln_dev(du)
+                alpha *= 0.5
+                new = residual_norm()
+            norm = new
+            alphas.append(alpha)
             it += 1
             if log:
-                log(f"  newton {it}: {k} GMRES iterations, ||R|| = {norm:.3e} ({time.perf_counter() - t0:.1f} s)")
+                log(f"  newton {it}: {k} GMRES iterations, " + (f"step {alpha:g}, " if alpha < 1.0 else "")
+                    + f"||R|| = {norm:.3e} ({time.perf_counter() - t0:.1f} s)")
             if norm < tol:
                 break
         g.sync()
         self.newton_info = {"iters": it, "linear_iters": lin, "norm": norm, "converged": bool(norm < tol),
-                            "seconds": time.perf_counter() - t0}
+                            "seconds": time.perf_counter() - t0, "steps": alphas}
         return self.newton_info
 
     def qoi(self) -> float:
@@ -448,7 +471,8 @@ def run_cycles(p: Problem, method: str, cycles: int, final_elems: int, net: Lear
         info = m.newton(log=log)
         J = m.qoi()
         rec = {"cycle": c, "elems": m.ne, "J": J, "newton": info["iters"], "converged": info["converged"],
-               "newton_seconds": round(info["seconds"], 4), "linear_iters": info["linear_iters"]}
+               "newton_seconds": round(info["seconds"], 4), "linear_iters": info["linear_iters"],
+               "damped_steps": sum(1 for a in info["steps"] if a < 1.0)}
         if c < cycles:
             if method == "nested":
                 d = nested_dwr(m)
