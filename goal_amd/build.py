@@ -57,6 +57,7 @@ SOURCES = [
     "goal_capi.cpp",
     "goal_assembly.hip",
     "goal_util.hip",
+    "goal_graph.hip",
     "goal_dwr.hip",
     "goal_solve.hip",
     "goal_learn.hip",

@@ -224,7 +224,8 @@ struct AmgHierarchy {
 struct goal_disc_s {
   goal::HostMesh mesh;
   std::vector<int64_t> node_adj_ptr;
-  std::vector<int32_t> node_adj;
+  std::vector<int32_t> node_adj;          // host copy, made on demand (host_node_adj)
+  DevBuf<int32_t> node_adj_dev;           // the columns as built on the device
   std::unique_ptr<AmgHierarchy> amg;
   int max_degree = 0;
   int64_t num_dofs = 0, nnz = 0;
@@ -323,53 +324,119 @@ struct goal_nested_s {
 
 namespace {
 
+// The discretization graph: on the device (radix sorts, goal_graph.hip) when
+// the element-pair count fits the sorts' 32-bit item counts, else on the
+// host (goal::build_graph); both give the same arrays bit for bit.  The host
+// keeps the node adjacency's row pointers (gather plan); the column lists are
+// copied to the host only when the multigrid set-up asks for them.
 void disc_build(goal_disc d) {
-  goal::HostGraph g = goal::build_graph(d->mesh, host_threads());
-  d->node_adj_ptr = g.node_adj_ptr;
-  d->amg.reset();
-  d->max_degree = g.max_degree;
-  d->num_dofs = g.num_dofs;
-  d->nnz = g.nnz;
   const goal::HostMesh& m = d->mesh;
+  const int npe = m.nodes_per_elem;
+  const int64_t nn = m.num_nodes, ne = m.num_elems, ns = ne * npe, np = ns * npe;
+  d->amg.reset();
   d->coords.upload(m.coords.data(), (int64_t)m.coords.size());
   d->conn.upload(m.conn.data(), (int64_t)m.conn.size());
   d->elem_set.upload(m.elem_set.data(), (int64_t)m.elem_set.size());
   d->vertex_nodes.upload(m.vertex_nodes.data(), (int64_t)m.vertex_nodes.size());
-  d->pos.upload(g.pos.data(), (int64_t)g.pos.size());
-  d->inc_ptr.upload(g.inc_ptr.data(), (int64_t)g.inc_ptr.size());
-  d->inc.upload(g.inc.data(), (int64_t)g.inc.size());
-  d->rec.upload(g.rec.data(), (int64_t)g.rec.size());
+  require(ns <= (int64_t)INT32_MAX, "graph: too many element-node incidences");
+  DevBuf<int64_t> adj_ptr;
+  DevBuf<int32_t> adj;
+  d->node_adj.clear();
+  if (np < (int64_t)INT32_MAX && nn < ((int64_t)1 << 30)) {
+    size_t tb = 0;
+    HIPCHK(goal::graph_temp_bytes(ne, npe, nn, &tb));
+    DevBuf<uint8_t> temp;
+    temp.alloc((int64_t)tb + 256);
+    int64_t nu = 0;
+    {
+      DevBuf<uint64_t> keys, sorted;
+      DevBuf<int64_t> count;
+      keys.alloc(np);
+      sorted.alloc(np);
+      count.alloc(1);
+      HIPCHK(goal::graph_adjacency(d->conn.p, ne, npe, nn, temp.p, tb, keys.p, sorted.p, count.p, stream()));
+      count.download(&nu, 1);
+      adj_ptr.alloc(nn + 1);
+      adj.alloc(nu);
+      HIPCHK(goal::graph_rows(keys.p, nu, nn, adj_ptr.p, adj.p, stream()));
+      sync();  // the key buffers are released here
+    }
+    {
+      DevBuf<int32_t> node_keys, slots;
+      node_keys.alloc(ns);
+      slots.alloc(ns);
+      d->inc.alloc(ns);
+      d->inc_ptr.alloc(nn + 1);
+      HIPCHK(goal::graph_incidence(d->conn.p, ne, npe, nn, temp.p, tb, node_keys.p, slots.p, d->inc.p, d->inc_ptr.p,
+                                   stream()));
+      d->pos.alloc(np);
+      d->rec.alloc(ns);
+      d->posi.alloc(ns * npe);
+      HIPCHK(goal::graph_positions(d->conn.p, ne, npe, adj_ptr.p, adj.p, d->inc.p, d->pos.p, d->rec.p, d->posi.p,
+                                   stream()));
+      sync();
+    }
+    d->node_adj_ptr.resize(nn + 1);
+    adj_ptr.download(d->node_adj_ptr.data(), nn + 1);
+    d->max_degree = 0;
+    for (int64_t a = 0; a < nn; ++a)
+      d->max_degree = std::max(d->max_degree, (int)(d->node_adj_ptr[a + 1] - d->node_adj_ptr[a]));
+  } else {
+    goal::HostGraph g = goal::build_graph(m, host_threads());
+    d->node_adj_ptr = g.node_adj_ptr;
+    d->node_adj = std::move(g.node_adj);
+    d->max_degree = g.max_degree;
+    d->pos.upload(g.pos.data(), (int64_t)g.pos.size());
+    d->inc_ptr.upload(g.inc_ptr.data(), (int64_t)g.inc_ptr.size());
+    d->inc.upload(g.inc.data(), (int64_t)g.inc.size());
+    d->rec.upload(g.rec.data(), (int64_t)g.rec.size());
+    d->posi.upload(g.posi.data(), (int64_t)g.posi.size());
+    adj_ptr.upload(d->node_adj_ptr.data(), (int64_t)d->node_adj_ptr.size());
+    adj.upload(d->node_adj.data(), (int64_t)d->node_adj.size());
+  }
+  require(d->max_degree <= 65535, "graph: node degree exceeds 65535");
+  d->num_dofs = nn * goal::kNumEqs;
+  d->nnz = d->node_adj_ptr[nn] * goal::kNumEqs * goal::kNumEqs;
   {
     // 16 * deg(a) CSR values per node; at least 2048 per run, and always one node
-    d->gather_cap = std::max(2048, 16 * g.max_degree);
+    d->gather_cap = std::max(2048, 16 * d->max_degree);
     const int nmax = goal::gather_nodes_per_range();
     std::vector<int32_t> r{0};
     int64_t cur = 0;
-    for (int64_t a = 0; a < m.num_nodes; ++a) {
-      const int64_t v = 16 * (g.node_adj_ptr[a + 1] - g.node_adj_ptr[a]);
+    for (int64_t a = 0; a < nn; ++a) {
+      const int64_t v = 16 * (d->node_adj_ptr[a + 1] - d->node_adj_ptr[a]);
       if (a > r.back() && (cur + v > d->gather_cap || a - r.back() >= nmax)) {
         r.push_back((int32_t)a);
         cur = 0;
       }
       cur += v;
     }
-    r.push_back((int32_t)m.num_nodes);
+    r.push_back((int32_t)nn);
     d->num_ranges = (int64_t)r.size() - 1;
     d->ranges.upload(r.data(), (int64_t)r.size());
     d->ranges_host = r;
   }
-  d->posi.upload(g.posi.data(), (int64_t)g.posi.size());
-  DevBuf<int64_t> adj_ptr;
-  DevBuf<int32_t> adj;
-  adj_ptr.upload(g.node_adj_ptr.data(), (int64_t)g.node_adj_ptr.size());
-  adj.upload(g.node_adj.data(), (int64_t)g.node_adj.size());
-  d->node_adj = std::move(g.node_adj);
   d->row_ptr.alloc(d->num_dofs + 1);
   d->cols.alloc(d->nnz);
-  HIPCHK(goal::launch_expand_csr(adj_ptr.p, adj.p, m.num_nodes, d->row_ptr.p, d->cols.p, stream()));
+  HIPCHK(goal::launch_expand_csr(adj_ptr.p, adj.p, nn, d->row_ptr.p, d->cols.p, stream()));
   d->x.alloc(d->num_dofs);
   HIPCHK(goal::launch_fill(d->x.p, d->num_dofs, 0.0, stream()));
-  sync();  // adj buffers are released on return
+  if (d->node_adj.empty() && adj.n > 0) {  // keep the columns on the device for the multigrid set-up
+    d->node_adj_dev.release();
+    std::swap(d->node_adj_dev.p, adj.p);
+    std::swap(d->node_adj_dev.n, adj.n);
+  }
+  sync();  // the graph buffers are released on return
+}
+
+// The node adjacency's column lists on the host (the multigrid aggregation
+// runs there); copied from the device on first use.
+const std::vector<int32_t>& host_node_adj(goal_disc d) {
+  if (d->node_adj.empty() && d->node_adj_dev.n > 0) {
+    d->node_adj.resize(d->node_adj_dev.n);
+    d->node_adj_dev.download(d->node_adj.data(), d->node_adj_dev.n);
+  }
+  return d->node_adj;
 }
 
 goal_disc new_disc(goal::HostMesh&& m) {
@@ -647,7 +714,7 @@ void amg_build_structure(goal_disc d) {
   auto h = std::make_unique<AmgHierarchy>();
   const int64_t nn0 = d->mesh.num_nodes;
   std::vector<int64_t> fptr = d->node_adj_ptr;
-  std::vector<int32_t> fcol = d->node_adj;
+  std::vector<int32_t> fcol = host_node_adj(d);
   int64_t nf = nn0;
   {  // row node of every finest-level block
     std::vector<int32_t> brow(fptr[nf]);

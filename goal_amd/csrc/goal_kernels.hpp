@@ -172,6 +172,17 @@ hipError_t launch_rsub(int64_t n, const double* b, double* y, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 hipError_t launch_axpy(double* y, const double* x, double a, int64_t n, hipStream_t s);
+// Discretization graph on the device (goal_graph.hip): temp_bytes sizes the
+// radix-sort scratch for all three steps; graph_adjacency leaves the unique
+// node-pair keys (count in *num_unique, on the device) in `keys`.
+hipError_t graph_temp_bytes(int64_t ne, int npe, int64_t nn, size_t* bytes);
+hipError_t graph_adjacency(const int32_t* conn, int64_t ne, int npe, int64_t nn, void* temp, size_t temp_bytes,
+                           uint64_t* keys, uint64_t* keys_sorted, int64_t* num_unique, hipStream_t s);
+hipError_t graph_rows(const uint64_t* ukeys, int64_t nu, int64_t nn, int64_t* adj_ptr, int32_t* adj, hipStream_t s);
+hipError_t graph_incidence(const int32_t* conn, int64_t ne, int npe, int64_t nn, void* temp, size_t temp_bytes,
+                           int32_t* keys_out, int32_t* slots, int32_t* inc, int64_t* inc_ptr, hipStream_t s);
+hipError_t graph_positions(const int32_t* conn, int64_t ne, int npe, const int64_t* adj_ptr, const int32_t* adj,
+                           const int32_t* inc, uint16_t* pos, int32_t* rec, uint16_t* posi, hipStream_t s);
 hipError_t launch_expand_csr(const int64_t* node_adj_ptr, const int32_t* node_adj, int64_t num_nodes,
                              int64_t* row_ptr, int32_t* cols, hipStream_t s);
 // Deterministic two-pass reductions into out[0] (device).  scratch holds at
