@@ -22,7 +22,7 @@ ARCH = os.environ.get("GOAL_AMD_ARCH", "gfx950")
 # Minimum waves per SIMD requested for the element kernel (register cap).
 ELEM_WAVES = int(os.environ.get("GOAL_ELEM_WAVES", "3"))
 # Elements per workgroup of the matrix-core (neo-Hookean ten-node, 4-point) element kernel.
-MF_G = int(os.environ.get("GOAL_MF_G", "6"))
+MF_G = int(os.environ.get("GOAL_MF_G", "16"))
 # 0 builds the vector-ALU element-matrix phase for every kernel instead.
 MF = int(os.environ.get("GOAL_MF", "1"))
 

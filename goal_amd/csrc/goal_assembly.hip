@@ -89,12 +89,17 @@ struct QPStore<MODEL_NEOHOOKEAN, S> {
 // (the register budget); five-wave workgroups were only ever placed one per CU.
 //
 // Neo-Hookean ten-node tets with four points (MF) form their element matrices
-// on the matrix cores instead, in tiles of 16 columns per wavefront: G = 6
-// elements on four waves, their 18 tiles dealt round-robin (measured on the
-// P2 64^3 cube: FP64 element pass 7.04 ms at G = 6 against 7.69 ms at G = 4,
-// where every wave takes one element).
+// on the matrix cores instead, in tiles of 16 columns per wavefront: G = 16
+// elements on four waves, their 48 tiles dealt round-robin, the 64 points of
+// phase C1 filling wave 0.  The 77 KB of LDS this takes leave two workgroups
+// per CU (2 waves per SIMD, 215 registers).  Measured on the P2 64^3 cube,
+// FP64 element pass / FP32 one: G = 16 6.48 / 4.89 ms, 14 6.80 / 5.21,
+// 12 6.73 / 4.79, 8 7.32 / 5.66 and 6 7.87 / 6.10 at 2 waves per SIMD; at 3
+// waves per SIMD (168 registers) G = 6 6.89 / 5.22, 8 8.26 / 5.04 and
+// 10 8.47 / 4.97 ms.  Larger groups share phase C1's one-wave latency and the
+// barriers among more elements.
 #ifndef GOAL_MF_G
-#define GOAL_MF_G 6
+#define GOAL_MF_G 16
 #endif
 #ifndef GOAL_MF
 #define GOAL_MF 1  // 0: the vector-ALU phase E for every kernel
@@ -569,14 +574,15 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
 
     // ---- C1: quadrature-point state (Kinematics, model, Mixed; states), one lane per point
     if constexpr (PF) {
-      // wave 1 idles in this phase (the points take fewer lanes than one
-      // wave): it fetches the next group's connectivity, incidence records,
-      // vertex coordinates and DOFs, one (element, node) per lane
-      static_assert(G * NQ <= 64 && G * NN <= 64, "phase C1 on wave 0, the next group's nodes on wave 1");
+      // the waves past the first idle in this phase (the points take fewer
+      // lanes than one wave): they fetch the next group's connectivity,
+      // incidence records, vertex coordinates and DOFs, one (element, node)
+      // per lane
+      static_assert(G * NQ <= 64 && G * NN <= T - 64, "phase C1 on wave 0, the next group's nodes on the others");
       const int64_t gn = group_of(it + 1, ngroups, a.xcd_remap);
       fetched = gn >= 0;
       const int l = tid - 64;
-      if (fetched && l >= 0 && l < 64) {
+      if (fetched && l >= 0) {
         const int64_t en = (g_lo + gn) * G;
         if (l < (int)min((int64_t)G, ne - en) * NN) {
           const int el = l / NN, n = l % NN;
