@@ -101,6 +101,9 @@ struct QPStore<MODEL_NEOHOOKEAN, S> {
 #ifndef GOAL_MF_G
 #define GOAL_MF_G 16
 #endif
+#ifndef GOAL_MF_WAVES
+#define GOAL_MF_WAVES 2
+#endif
 #ifndef GOAL_MF
 #define GOAL_MF 1  // 0: the vector-ALU phase E for every kernel
 #endif
@@ -110,7 +113,7 @@ struct Layout {
   static constexpr bool MF = GOAL_MF && MODEL == MODEL_NEOHOOKEAN && NN == 10 && NQ == 4;
   static constexpr int G = MF ? GOAL_MF_G : (NN == 10) ? 6 : (NQ == 1 ? 16 : 4);
   static constexpr int T = MF ? 256 : ((G * ND + 63) / 64) * 64;
-  static constexpr int W = (MODEL == MODEL_J2) ? 1 : GOAL_ELEM_WAVES;
+  static constexpr int W = (MODEL == MODEL_J2) ? 1 : MF ? GOAL_MF_WAVES : GOAL_ELEM_WAVES;
 };
 
 template <int NN, int NQ, int G>
@@ -343,33 +346,16 @@ struct MfTiles {
   }
 };
 
+// The thirteen products of one tile and the stores of its values: lane l
+// holds the A operands (aN, aG) of row l % 16 at point g = l / 16 and the
+// point-tangent column C of column col (-1: a padding lane), and stores rows
+// a = M::row(g, r) of column col.
 template <int NN, bool TR, typename T>
-__device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const double* wdv,
-                                      const QPStore<MODEL_NEOHOOKEAN, T>* qs, const int32_t* rec, T* em, int lane,
-                                      int t) {
+__device__ inline void mf_product_store(T aN, T aG0, T aG1, T aG2, const T* C, int col, int g, const int32_t* rec,
+                                        T* em) {
   constexpr int ND = 4 * NN;
   using M = Mfma<T>;
   using acc = typename M::acc;
-  const int q = lane >> 4, n = lane & 15, g = lane >> 4;
-  T aN = (T)0, aG0 = (T)0, aG1 = (T)0, aG2 = (T)0;
-  if (n < NN) {
-    const double* sa = sh[q][n];
-    aN = (T)sa[0];
-    aG0 = (T)sa[1];
-    aG1 = (T)sa[2];
-    aG2 = (T)sa[3];
-  }
-  const int col = MfTiles<NN>::col(t, n);
-  T C[13];
-  if (col >= 0) {
-    const double w = wdv[q];
-    const double* shb = sh[q][col >> 2];
-    const T G[3] = {(T)(w * shb[1]), (T)(w * shb[2]), (T)(w * shb[3])};
-    dev::nh_direction<T>(qs[q].b, col & 3, (T)(w * shb[0]), G, C);
-  } else {
-#pragma unroll
-    for (int r = 0; r < 13; ++r) C[r] = (T)0;
-  }
   acc k0 = {0, 0, 0, 0}, k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
   k0 = M::mac(aG0, C[0], k0);
   k1 = M::mac(aG0, C[3], k1);
@@ -404,6 +390,82 @@ __device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const d
       }
     }
   }
+}
+
+template <int NN, bool TR, typename T>
+__device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const double* wdv,
+                                      const QPStore<MODEL_NEOHOOKEAN, T>* qs, const int32_t* rec, T* em, int lane,
+                                      int t) {
+  const int q = lane >> 4, n = lane & 15;
+  T aN = (T)0, aG0 = (T)0, aG1 = (T)0, aG2 = (T)0;
+  if (n < NN) {
+    const double* sa = sh[q][n];
+    aN = (T)sa[0];
+    aG0 = (T)sa[1];
+    aG1 = (T)sa[2];
+    aG2 = (T)sa[3];
+  }
+  const int col = MfTiles<NN>::col(t, n);
+  T C[13];
+  if (col >= 0) {
+    const double w = wdv[q];
+    const double* shb = sh[q][col >> 2];
+    const T G[3] = {(T)(w * shb[1]), (T)(w * shb[2]), (T)(w * shb[3])};
+    dev::nh_direction<T>(qs[q].b, col & 3, (T)(w * shb[0]), G, C);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 13; ++r) C[r] = (T)0;
+  }
+  mf_product_store<NN, TR, T>(aN, aG0, aG1, aG2, C, col, q, rec, em);
+}
+
+// The same tiles in software-pipelined form: a wave takes whole elements and,
+// for tile t, forms the point-tangent columns C of the next tile on the
+// vector ALUs while the 13 products of tile t run on the matrix cores.  The
+// column and the products are branch-free, so both sit in one scheduling
+// region: a padding lane evaluates a real column (or node row) instead, and
+// since column n of B and row m of A reach only column n and row m of the
+// product, which are not stored, what it holds is never seen.  The products
+// and their summation order are those of emit_tile_mfma, and so are the
+// bits of every stored value.
+//
+// Used for FP32 element matrices only.  Measured on the P2 64^3 cube (same
+// GPU, same run): FP32 element pass 5.10 -> 4.62 ms, but FP64 7.32 -> 8.62 ms:
+// on MI355X an FP64 16x16x4 product takes the FP64 vector rate (64 cycles),
+// and overlapping the two in one wave only slows the FP64 kernel.  Explicit
+// interleaving (sched_group_barrier: one product, then 1-2 LDS reads and
+// 6-12 vector instructions) measured 1-2% slower than the compiler's order.
+#ifndef GOAL_MF_PIPE
+#define GOAL_MF_PIPE 1
+#endif
+template <int NN, int TILE, typename T>
+__device__ inline void mf_tile_column(const double (*sh)[NN][kShapeRow], const double* wdv,
+                                      const QPStore<MODEL_NEOHOOKEAN, T>* qs, int lane, T* C) {
+  using Tiles = MfTiles<NN>;
+  const int q = lane >> 4, n = lane & 15;
+  const int col = Tiles::col(TILE, n);
+  const bool valid = col >= 0;
+  const int b = valid ? (col >> 2) : 0;
+  const double w = wdv[q];
+  const double* shb = sh[q][b];
+  const T G[3] = {(T)(w * shb[1]), (T)(w * shb[2]), (T)(w * shb[3])};
+  if constexpr (TILE < Tiles::DT) dev::nh_direction_u<T>(qs[q].b, valid ? (col & 3) : 0, G, C);
+  else dev::nh_direction_p<T>(qs[q].b, (T)(w * shb[0]), G, C);
+}
+
+// A operands of an element's products: N_q[n], G_q[n][J] of lane l = 16 q + n
+// (node 0's past the element's nodes: padding rows).
+template <int NN, typename T>
+__device__ inline void mf_rows(const double (*sh)[NN][kShapeRow], int lane, T* A) {
+  const int q = lane >> 4, n = lane & 15;
+  const double* sa = sh[q][n < NN ? n : 0];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) A[j] = (T)sa[j];
+}
+
+template <int NN, int TILE, bool TR, typename T>
+__device__ inline void mf_tile_product(const T* A, const T* C, const int32_t* rec, T* em, int lane) {
+  mf_product_store<NN, TR, T>(A[0], A[1], A[2], A[3], C, MfTiles<NN>::col(TILE, lane & 15), lane >> 4, rec, em);
 }
 
 template <int NN, int NQ, int MODEL, int KIND, typename TE>
@@ -775,11 +837,33 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
     if constexpr (std::is_same<TE, float>::value) em = a.em32;
     else em = a.em;
     if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN && Lay::MF) {
-      // matrix cores: the (element, tile) tasks dealt round-robin over the waves
-      constexpr int NTL = MfTiles<NN>::NT;
-      for (int task = tid >> 6; task < nv * NTL; task += T / 64) {
-        const int el = task / NTL;
-        emit_tile_mfma<NN, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], s.rec[el], em, tid & 63, task % NTL);
+      if constexpr (GOAL_MF_PIPE && std::is_same<TE, float>::value) {
+        // matrix cores, pipelined: wave w takes elements w, w + 4, ... and
+        // forms each tile's columns during the previous tile's products
+        static_assert(MfTiles<NN>::DT == 2 && MfTiles<NN>::NT == 3, "two displacement tiles and one pressure tile");
+        constexpr int NW = T / 64;
+        const int w = tid >> 6, lane = tid & 63;
+        if (w < nv) {
+          TE C0[13], C1[13], C2[13], A[4];
+          mf_tile_column<NN, 0, TE>(s.sh[w], s.wdv[w], sm.ext.qs[w], lane, C0);
+          for (int el = w; el < nv; el += NW) {
+            const int en = min(el + NW, nv - 1);
+            mf_rows<NN, TE>(s.sh[el], lane, A);
+            mf_tile_column<NN, 1, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], lane, C1);
+            mf_tile_product<NN, 0, TR, TE>(A, C0, s.rec[el], em, lane);
+            mf_tile_column<NN, 2, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], lane, C2);
+            mf_tile_product<NN, 1, TR, TE>(A, C1, s.rec[el], em, lane);
+            mf_tile_column<NN, 0, TE>(s.sh[en], s.wdv[en], sm.ext.qs[en], lane, C0);
+            mf_tile_product<NN, 2, TR, TE>(A, C2, s.rec[el], em, lane);
+          }
+        }
+      } else {
+        // matrix cores: the (element, tile) tasks dealt round-robin over the waves
+        constexpr int NTL = MfTiles<NN>::NT;
+        for (int task = tid >> 6; task < nv * NTL; task += T / 64) {
+          const int el = task / NTL;
+          emit_tile_mfma<NN, TR, TE>(s.sh[el], s.wdv[el], sm.ext.qs[el], s.rec[el], em, tid & 63, task % NTL);
+        }
       }
     } else if constexpr (JAC && MODEL == MODEL_NEOHOOKEAN) {
       if constexpr (T == 256 && G * NN <= 64 && std::is_same<TE, double>::value) {

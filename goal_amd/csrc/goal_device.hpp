@@ -532,38 +532,48 @@ __host__ __device__ inline void nh_base(const QP& q, const NeoHookean& m, const 
 // These are the columns of the point tangent summed against G; scaling G and
 // N_b by w dv scales the result.  T = float evaluates the FP32 tangent from the
 // FP64 point state.
+// Displacement DOF k (< 3) of node b.
+template <typename T, typename S>
+__host__ __device__ inline void nh_direction_u(const NHBaseT<S>& b, int k, const T* G, T* C) {
+  const T c = (T)b.c, tJ = (T)(b.tau * b.J);
+  T FG[3], PG[3], CG[3];
+  for (int i = 0; i < 3; ++i) {
+    FG[i] = (T)b.F[3 * i] * G[0] + (T)b.F[3 * i + 1] * G[1] + (T)b.F[3 * i + 2] * G[2];
+    PG[i] = (T)b.P[3 * i] * G[0] + (T)b.P[3 * i + 1] * G[1] + (T)b.P[3 * i + 2] * G[2];
+    CG[i] = (T)b.Cinv[3 * i] * G[0] + (T)b.Cinv[3 * i + 1] * G[1] + (T)b.Cinv[3 * i + 2] * G[2];
+  }
+  // k-indexed quantities come from LDS (computed address), not from
+  // register arrays, so no per-lane selects
+  const T gk = (T)b.Fi[k] * G[0] + (T)b.Fi[3 + k] * G[1] + (T)b.Fi[6 + k] * G[2];
+  const T FGk = (T)b.F[3 * k] * G[0] + (T)b.F[3 * k + 1] * G[1] + (T)b.F[3 * k + 2] * G[2];
+  const T u = (T)(2.0 / 3.0) * c * FGk;
+  for (int i = 0; i < 3; ++i) {
+    const T t = c * FG[i] - PG[i];
+    const T dik = (i == k) ? c : (T)0;
+    for (int jj = 0; jj < 3; ++jj)
+      C[3 * i + jj] = gk * (T)b.Q[3 * i + jj] + (T)b.Fi[3 * jj + k] * t - u * (T)b.Fi[3 * jj + i] + dik * G[jj];
+  }
+  C[9] = (T)b.beta * gk;
+  const T vG = (T)b.v[0] * G[0] + (T)b.v[1] * G[1] + (T)b.v[2] * G[2];
+  for (int j = 0; j < 3; ++j) C[10 + j] = tJ * (gk * (T)b.v[j] - (T)b.h[k] * CG[j] - (T)b.Fi[3 * j + k] * vG);
+}
+
+// Pressure DOF of node b.
+template <typename T, typename S>
+__host__ __device__ inline void nh_direction_p(const NHBaseT<S>& b, T N, const T* G, T* C) {
+  const T J = (T)b.J, tJ = (T)(b.tau * b.J);
+  const T NJ = N * J;
+  for (int i = 0; i < 3; ++i)
+    for (int jj = 0; jj < 3; ++jj) C[3 * i + jj] = NJ * (T)b.Fi[3 * jj + i];
+  C[9] = N * (T)b.ikappa;
+  for (int j = 0; j < 3; ++j)
+    C[10 + j] = tJ * ((T)b.Cinv[3 * j] * G[0] + (T)b.Cinv[3 * j + 1] * G[1] + (T)b.Cinv[3 * j + 2] * G[2]);
+}
+
 template <typename T, typename S>
 __host__ __device__ inline void nh_direction(const NHBaseT<S>& b, int k, T N, const T* G, T* C) {
-  const T c = (T)b.c, J = (T)b.J, tJ = (T)(b.tau * b.J);
-  if (k < 3) {
-    T FG[3], PG[3], CG[3];
-    for (int i = 0; i < 3; ++i) {
-      FG[i] = (T)b.F[3 * i] * G[0] + (T)b.F[3 * i + 1] * G[1] + (T)b.F[3 * i + 2] * G[2];
-      PG[i] = (T)b.P[3 * i] * G[0] + (T)b.P[3 * i + 1] * G[1] + (T)b.P[3 * i + 2] * G[2];
-      CG[i] = (T)b.Cinv[3 * i] * G[0] + (T)b.Cinv[3 * i + 1] * G[1] + (T)b.Cinv[3 * i + 2] * G[2];
-    }
-    // k-indexed quantities come from LDS (computed address), not from
-    // register arrays, so no per-lane selects
-    const T gk = (T)b.Fi[k] * G[0] + (T)b.Fi[3 + k] * G[1] + (T)b.Fi[6 + k] * G[2];
-    const T FGk = (T)b.F[3 * k] * G[0] + (T)b.F[3 * k + 1] * G[1] + (T)b.F[3 * k + 2] * G[2];
-    const T u = (T)(2.0 / 3.0) * c * FGk;
-    for (int i = 0; i < 3; ++i) {
-      const T t = c * FG[i] - PG[i];
-      const T dik = (i == k) ? c : (T)0;
-      for (int jj = 0; jj < 3; ++jj)
-        C[3 * i + jj] = gk * (T)b.Q[3 * i + jj] + (T)b.Fi[3 * jj + k] * t - u * (T)b.Fi[3 * jj + i] + dik * G[jj];
-    }
-    C[9] = (T)b.beta * gk;
-    const T vG = (T)b.v[0] * G[0] + (T)b.v[1] * G[1] + (T)b.v[2] * G[2];
-    for (int j = 0; j < 3; ++j) C[10 + j] = tJ * (gk * (T)b.v[j] - (T)b.h[k] * CG[j] - (T)b.Fi[3 * j + k] * vG);
-  } else {
-    const T NJ = N * J;
-    for (int i = 0; i < 3; ++i)
-      for (int jj = 0; jj < 3; ++jj) C[3 * i + jj] = NJ * (T)b.Fi[3 * jj + i];
-    C[9] = N * (T)b.ikappa;
-    for (int j = 0; j < 3; ++j)
-      C[10 + j] = tJ * ((T)b.Cinv[3 * j] * G[0] + (T)b.Cinv[3 * j + 1] * G[1] + (T)b.Cinv[3 * j + 2] * G[2]);
-  }
+  if (k < 3) nh_direction_u<T>(b, k, G, C);
+  else nh_direction_p<T>(b, N, G, C);
 }
 
 // tau = 1/2 c0 h^2 / mu with h^2 the mean squared edge length
