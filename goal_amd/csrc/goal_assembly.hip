@@ -101,6 +101,13 @@ struct QPStore<MODEL_NEOHOOKEAN, S> {
 #ifndef GOAL_MF_G
 #define GOAL_MF_G 16
 #endif
+// Four-node tets with one point: G = 32 (512 lanes, 60 KB of LDS, two
+// workgroups per CU at 4 waves per SIMD).  Measured on the P1 J2 bar
+// (point + element pass): 6.62 ms at G = 32, 6.76-6.80 at 16, 7.49 at 8 and
+// 9.52 at 24 (three 6-wave workgroups per CU).
+#ifndef GOAL_P1_G
+#define GOAL_P1_G 32
+#endif
 #ifndef GOAL_MF_WAVES
 #define GOAL_MF_WAVES 2
 #endif
@@ -111,7 +118,7 @@ template <int NN, int NQ, int MODEL>
 struct Layout {
   static constexpr int ND = 4 * NN;
   static constexpr bool MF = GOAL_MF && MODEL == MODEL_NEOHOOKEAN && NN == 10 && NQ == 4;
-  static constexpr int G = MF ? GOAL_MF_G : (NN == 10) ? 6 : (NQ == 1 ? 16 : 4);
+  static constexpr int G = MF ? GOAL_MF_G : (NN == 10) ? 6 : (NQ == 1 ? GOAL_P1_G : 4);
   static constexpr int T = MF ? 256 : ((G * ND + 63) / 64) * 64;
   static constexpr int W = (MODEL == MODEL_J2) ? 1 : MF ? GOAL_MF_WAVES : GOAL_ELEM_WAVES;
 };
