@@ -15,8 +15,8 @@
 // writes the rows once.  No atomics, no zero fill, and the summation order is
 // fixed, so results are bitwise reproducible.
 //
-// Pass 1 work layout: a workgroup takes G elements at a time (6 ten-node or
-// 16 four-node tets) and one lane per column of their element matrices.  The
+// Pass 1 work layout: a workgroup takes G elements at a time (6 to 32, by
+// element kind) and one lane per column of their element matrices.  The
 // phases, separated by workgroup barriers, keep all element data in LDS and
 // hand out their tasks over all lanes:
 //   A  gather connectivity, vertex coordinates, DOFs        (element, DOF)
@@ -84,9 +84,10 @@ struct QPStore<MODEL_NEOHOOKEAN, S> {
 
 // Workgroup shape.  Phase E gives one lane to each column of an element
 // matrix, so a workgroup takes G elements and G*ND lanes, rounded up to whole
-// wavefronts: 6 ten-node tets on 256 lanes, 16 four-node tets on 256.  Four
-// waves per workgroup let three workgroups share a CU at W = 3 waves per SIMD
-// (the register budget); five-wave workgroups were only ever placed one per CU.
+// wavefronts: 6 ten-node tets on 256 lanes, 32 four-node one-point tets on
+// 512, 4 four-node four-point tets on 64.  Four-wave ten-node workgroups let
+// three workgroups share a CU at W = 3 waves per SIMD (the register budget);
+// five-wave workgroups were only ever placed one per CU.
 //
 // Neo-Hookean ten-node tets with four points (MF) form their element matrices
 // on the matrix cores instead, in tiles of 16 columns per wavefront: G = 16
