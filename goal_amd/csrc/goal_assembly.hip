@@ -914,17 +914,34 @@ __global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
 // (records inc_ptr[a] .. inc_ptr[a+1]); for a run of consecutive nodes both
 // stay contiguous.  A workgroup takes one run (planned on the host so its CSR
 // values fit the LDS budget `cap`), clears an LDS copy of those values, streams
-// the run's blocks with U loads per lane in flight and the next batch
-// prefetched, adds each value at its slot with LDS atomics, then stores the
-// rows once.  Wave i of the four handles row i (equation i) of every node, so
-// no two waves touch the same slot and each slot is summed in one wave's fixed
-// instruction and lane order: results are bitwise reproducible.  No atomics
-// to HBM, no zero fill of the CSR values.
+// the run's blocks in column pairs, U pair loads per lane in flight and the
+// next batch prefetched, adds each value at its slot with LDS atomics, then
+// stores the rows once.  Wave i of the four handles row i (equation i) of
+// every node, so no two waves touch the same slot and each slot is summed in
+// one wave's fixed instruction and lane order: results are bitwise
+// reproducible.  No atomics to HBM, no zero fill of the CSR values.
+// Column-pair loads, U per lane per batch (measured on the P2 64^3 cube,
+// same GPU, gather FP64 / FP32: single-column loads with U = 2 / 4 5.79 /
+// 4.22 ms; pairs with U = 1 / 2 5.64 / 4.09; pairs U = 2 / 4 5.70 / 4.25;
+// four columns per lane, U = 1 / 1 5.72 / 4.58).
 #ifndef GOAL_GATHER_U64
-#define GOAL_GATHER_U64 2
+#define GOAL_GATHER_U64 1
 #endif
 #ifndef GOAL_GATHER_U32
-#define GOAL_GATHER_U32 4
+#define GOAL_GATHER_U32 2
+#endif
+// Four-node tets (16-column rows) keep single-column loads: measured on the
+// P1 J2 bar (gather FP64 / FP32), single loads with U = 2 / 4 2.74-2.76 /
+// 1.93-1.95 ms, U = 4 / 8 2.76 / 2.19; pairs with U = 1 / 2 3.16 / 2.01,
+// U = 2 / 4 2.81 / 2.04.
+#ifndef GOAL_GATHER_K_P1
+#define GOAL_GATHER_K_P1 1
+#endif
+#ifndef GOAL_GATHER_U64_P1
+#define GOAL_GATHER_U64_P1 2
+#endif
+#ifndef GOAL_GATHER_U32_P1
+#define GOAL_GATHER_U32_P1 4
 #endif
 constexpr int kGatherThreads = 256;  // four waves: one per equation
 constexpr int kGatherRecords = 256;  // incidence records staged per chunk
@@ -933,7 +950,12 @@ constexpr int kGatherNodes = 64;     // nodes per run, at most
 template <int NN, typename TE>
 __global__ __launch_bounds__(kGatherThreads) void gather_kernel(GatherArgs g) {
   constexpr int ND = 4 * NN, RB = 4 * ND, NT = kGatherThreads, RC = kGatherRecords, NC = kGatherNodes;
-  constexpr int U = sizeof(TE) == 8 ? GOAL_GATHER_U64 : GOAL_GATHER_U32;  // loads per lane per batch
+  constexpr bool P2 = NN == 10;
+  constexpr int KV = P2 ? 2 : GOAL_GATHER_K_P1;  // adjacent columns per load
+  constexpr int U = sizeof(TE) == 8 ? (P2 ? GOAL_GATHER_U64 : GOAL_GATHER_U64_P1)
+                                    : (P2 ? GOAL_GATHER_U32 : GOAL_GATHER_U32_P1);  // loads per lane per batch
+  constexpr int NP = ND / KV;                                                       // loads per row
+  using V2 = TE __attribute__((ext_vector_type(KV)));
   extern __shared__ double acc[];
   __shared__ int64_t srow[4 * NC + 1];
   __shared__ int64_t sinc[NC + 1];
@@ -963,15 +985,17 @@ __global__ __launch_bounds__(kGatherThreads) void gather_kernel(GatherArgs g) {
         if constexpr (std::is_same<TE, float>::value) em = g.em32 + kb * RB + eq * ND;
         else em = g.em + kb * RB + eq * ND;
       }
-      const int total = g.vals ? m * ND : 0;
+      // a lane loads KV adjacent columns of one row (the row and record
+      // strides keep them aligned); q counts loads
+      const int total = g.vals ? m * NP : 0;
       // loads are unconditional (index clamped into the chunk) so all U issue
       // back to back; out-of-range lanes are dropped at the add
-      TE src[U];
+      V2 src[U];
       if (total > 0)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int q = min(u * 64 + lane, total - 1);
-          src[u] = em[(q / ND) * RB + q % ND];
+          src[u] = *reinterpret_cast<const V2*>(em + (q / NP) * RB + KV * (q % NP));
         }
       if (g.vals) {
         // kb * NN and m * NN are even (NN is 4 or 10): whole 32-bit words
@@ -1004,26 +1028,27 @@ __global__ __launch_bounds__(kGatherThreads) void gather_kernel(GatherArgs g) {
       }
       // double-buffered stream: batch A is summed while batch B is in flight
       // and the other way round (no register copies of loads in flight)
-      auto add_batch = [&](const TE* buf, int qb) {
+      auto add_batch = [&](const V2* buf, int qb) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int q = qb + u * 64 + lane;
           if (q < total) {
-            const int j = q / ND, c = q % ND;
+            const int j = q / NP, c = KV * (q % NP);  // the KV columns lie in node c / 4
             const int slot = (int)(srow[4 * rnode[j] + eq] - r0) + 4 * (int)spos[j * NN + (c >> 2)] + (c & 3);
-            atomicAdd(&acc[slot], (double)buf[u]);
+#pragma unroll
+            for (int k = 0; k < KV; ++k) atomicAdd(&acc[slot + k], (double)buf[u][k]);
           }
         }
       };
-      auto load_batch = [&](TE* buf, int qb) {
+      auto load_batch = [&](V2* buf, int qb) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int q = min(qb + u * 64 + lane, total - 1);
-          buf[u] = em[(q / ND) * RB + q % ND];
+          buf[u] = *reinterpret_cast<const V2*>(em + (q / NP) * RB + KV * (q % NP));
         }
       };
       for (int q0 = 0; q0 < total; q0 += 2 * U * 64) {
-        TE nxt[U];
+        V2 nxt[U];
         load_batch(nxt, q0 + U * 64);
         add_batch(src, q0);
         if (q0 + U * 64 >= total) break;
