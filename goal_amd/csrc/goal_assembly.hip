@@ -443,10 +443,13 @@ __device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const d
 // and overlapping the two in one wave only slows the FP64 kernel.  Explicit
 // interleaving (sched_group_barrier: one product, then 1-2 LDS reads and
 // 6-12 vector instructions) measured 1-2% slower than the compiler's order.
-// The FP64 kernel keeps the round-robin deal, whose waves run different tile
-// kinds at once: dealing each wave whole elements (its three tiles in a row,
-// unpipelined) took 7.34 -> 9.93 ms, and the pipelined loop over the
-// round-robin tasks 7.32 -> 8.49 ms (FP32 4.64 -> 4.74 ms).
+// The FP64 kernel keeps the round-robin deal, whose four waves work on the
+// tiles of one or two elements at once and so write adjacent pieces of the
+// same block rows together (the L2 merges them into whole lines): dealing
+// each wave whole elements (its three tiles in a row, unpipelined) took
+// 7.34 -> 9.93 ms, 10.3 ms with the tile kinds staggered across the waves
+// (HBM writes 24.5 -> 31.0 GB), and the pipelined loop over the round-robin
+// tasks 7.32 -> 8.49 ms (FP32 4.64 -> 4.74 ms).
 #ifndef GOAL_MF_PIPE
 #define GOAL_MF_PIPE 1
 #endif
