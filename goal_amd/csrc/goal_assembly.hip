@@ -191,6 +191,19 @@ struct Smem : std::conditional<kPrefetch<NN, NQ, MODEL, KIND, TE>, SPrefetch<NN,
   SBase<NN, NQ, G> b;
   Ext ext;
 };
+// Waves per SIMD the element kernel is compiled for: the register budget
+// (W, or GOAL_ELEM_WAVES for the J2 point-pass kinds), lowered to what the
+// workgroup's LDS allows (160 KB per CU), so the compiler is not held to an
+// occupancy the LDS rules out.
+template <int NN, int NQ, int MODEL, int KIND, typename TE>
+constexpr int elem_waves() {
+  using Lay = Layout<NN, NQ, MODEL>;
+  constexpr int want = kPointPass<MODEL, KIND> ? GOAL_ELEM_WAVES : Lay::W;
+  constexpr int per_cu = 163840 / (int)sizeof(Smem<NN, NQ, MODEL, KIND, Lay::G, TE>);
+  constexpr int lds = per_cu * (Lay::T / 64) / 4;
+  return lds < 1 ? 1 : (lds < want ? lds : want);
+}
+
 // Element-group schedule.  Plain: group = block + it * gridDim.  XCD slabs:
 // workgroups are dispatched round-robin over the 8 XCDs, so block b runs on
 // XCD b % 8; XCD x is given the groups [x*per, (x+1)*per) and its blocks walk
@@ -484,8 +497,7 @@ __device__ inline void mf_tile_product(const T* A, const T* C, const int32_t* re
 }
 
 template <int NN, int NQ, int MODEL, int KIND, typename TE>
-__global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T),
-                             (kPointPass<MODEL, KIND> ? GOAL_ELEM_WAVES : Layout<NN, NQ, MODEL>::W)) void element_kernel(
+__global__ __launch_bounds__((Layout<NN, NQ, MODEL>::T), (elem_waves<NN, NQ, MODEL, KIND, TE>())) void element_kernel(
     AsmArgs a) {
   using Lay = Layout<NN, NQ, MODEL>;
   using M = typename ModelOf<MODEL>::type;
