@@ -461,8 +461,9 @@ __device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const d
 // same block rows together (the L2 merges them into whole lines): dealing
 // each wave whole elements (its three tiles in a row, unpipelined) took
 // 7.34 -> 9.93 ms, 10.3 ms with the tile kinds staggered across the waves
-// (HBM writes 24.5 -> 31.0 GB), and the pipelined loop over the round-robin
-// tasks 7.32 -> 8.49 ms (FP32 4.64 -> 4.74 ms).
+// (HBM writes 24.5 -> 31.0 GB), waves 0-2 on one element's three tiles and
+// wave 3 on a fourth element 10.5 ms (33.9 GB), and the pipelined loop over
+// the round-robin tasks 7.32 -> 8.49 ms (FP32 4.64 -> 4.74 ms).
 #ifndef GOAL_MF_PIPE
 #define GOAL_MF_PIPE 1
 #endif
