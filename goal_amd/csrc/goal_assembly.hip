@@ -456,9 +456,8 @@ __device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const d
 // and overlapping the two in one wave only slows the FP64 kernel.  Explicit
 // interleaving (sched_group_barrier: one product, then 1-2 LDS reads and
 // 6-12 vector instructions) measured 1-2% slower than the compiler's order.
-// The FP64 kernel keeps the round-robin deal, whose four waves work on the
-// tiles of one or two elements at once and so write adjacent pieces of the
-// same block rows together (the L2 merges them into whole lines): dealing
+// The FP64 kernel keeps the round-robin deal, the fastest measured and the
+// one with the fewest HBM writes (24.5 GB for 20.6 GB of blocks): dealing
 // each wave whole elements (its three tiles in a row, unpipelined) took
 // 7.34 -> 9.93 ms, 10.3 ms with the tile kinds staggered across the waves
 // (HBM writes 24.5 -> 31.0 GB), waves 0-2 on one element's three tiles and
