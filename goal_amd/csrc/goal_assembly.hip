@@ -348,14 +348,26 @@ struct Mfma<float> {
   __device__ static int row(int g, int r) { return 4 * g + r; }
 };
 
-// Tiles of the element's columns: displacement columns of five nodes (15
-// lanes) per tile, then the pressure columns (up to 16 nodes per tile), so
-// every lane of a tile takes the same branch of dev::nh_direction.
+// Tiles of the element's columns, two ways.
+//  - By kind (col): displacement columns of five nodes (15 lanes) per tile,
+//    then the pressure columns (up to 16 nodes per tile), so every lane of a
+//    tile takes the same branch of dev::nh_direction.  The pipelined FP32
+//    kernel needs this: its branch-free columns fix the kind per tile.
+//  - Contiguous (col_contig): columns 16 t .. 16 t + 15, so a tile stores
+//    whole 128-byte runs of each block row and the lanes of a tile take both
+//    branches.  The FP64 kernel uses these.  Measured on the P2 64^3 cube
+//    (same GPU): FP64 element pass 7.33-7.40 -> 7.22-7.24 ms, its HBM writes
+//    24.5 -> 21.6 GB (the blocks are 20.6 GB, residual rows and stresses 1 GB):
+//    with tiles by kind every 128-byte line of a block row took pieces from
+//    two tiles, and part of them reached HBM as partial writes.  Trading rows
+//    between neighbouring lanes to store two columns at once (16-byte stores)
+//    was slower: 7.75 against 7.14 ms.
 template <int NN>
 struct MfTiles {
   static constexpr int DT = (3 * NN + 14) / 15;
   static constexpr int PT = (NN + 15) / 16;
   static constexpr int NT = DT + PT;
+  static_assert(NT == (4 * NN + 15) / 16, "as many contiguous tiles as tiles by kind");
   // element-matrix column of lane n in tile t, or -1
   __device__ static int col(int t, int n) {
     if (t < DT) {
@@ -364,6 +376,10 @@ struct MfTiles {
     }
     const int b = 16 * (t - DT) + n;
     return b < NN ? 4 * b + 3 : -1;
+  }
+  __device__ static int col_contig(int t, int n) {
+    const int c = 16 * t + n;
+    return c < 4 * NN ? c : -1;
   }
 };
 
@@ -413,6 +429,8 @@ __device__ inline void mf_product_store(T aN, T aG0, T aG1, T aG2, const T* C, i
   }
 }
 
+// One tile of contiguous columns (MfTiles::col_contig): each lane evaluates
+// the column of its own kind (dev::nh_direction branches per lane).
 template <int NN, bool TR, typename T>
 __device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const double* wdv,
                                       const QPStore<MODEL_NEOHOOKEAN, T>* qs, const int32_t* rec, T* em, int lane,
@@ -426,7 +444,7 @@ __device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const d
     aG1 = (T)sa[2];
     aG2 = (T)sa[3];
   }
-  const int col = MfTiles<NN>::col(t, n);
+  const int col = MfTiles<NN>::col_contig(t, n);
   T C[13];
   if (col >= 0) {
     const double w = wdv[q];
@@ -456,8 +474,9 @@ __device__ inline void emit_tile_mfma(const double (*sh)[NN][kShapeRow], const d
 // and overlapping the two in one wave only slows the FP64 kernel.  Explicit
 // interleaving (sched_group_barrier: one product, then 1-2 LDS reads and
 // 6-12 vector instructions) measured 1-2% slower than the compiler's order.
-// The FP64 kernel keeps the round-robin deal, the fastest measured and the
-// one with the fewest HBM writes (24.5 GB for 20.6 GB of blocks): dealing
+// The FP64 kernel keeps the round-robin deal of contiguous tiles, the fastest
+// measured (with tiles by kind, 24.5 GB of HBM writes for 20.6 GB of
+// blocks; the other deals below were measured with tiles by kind): dealing
 // each wave whole elements (its three tiles in a row, unpipelined) took
 // 7.34 -> 9.93 ms, 10.3 ms with the tile kinds staggered across the waves
 // (HBM writes 24.5 -> 31.0 GB), waves 0-2 on one element's three tiles and
