@@ -93,7 +93,8 @@ struct QPStore<MODEL_NEOHOOKEAN, S> {
 // on the matrix cores instead, in tiles of 16 columns per wavefront: G = 16
 // elements on four waves, their 48 tiles dealt round-robin, the 64 points of
 // phase C1 filling wave 0.  The 77 KB of LDS this takes leave two workgroups
-// per CU (2 waves per SIMD, 215 registers).  Measured on the P2 64^3 cube,
+// per CU, 2 waves per SIMD (182 VGPRs for FP64 element matrices, 199 for
+// FP32).  Measured on the P2 64^3 cube,
 // FP64 element pass / FP32 one: G = 16 6.48 / 4.89 ms, 14 6.80 / 5.21,
 // 12 6.73 / 4.79, 8 7.32 / 5.66 and 6 7.87 / 6.10 at 2 waves per SIMD; at 3
 // waves per SIMD (168 registers) G = 6 6.89 / 5.22, 8 8.26 / 5.04 and
